@@ -1,0 +1,146 @@
+/*
+ * qsketch.h -- C ABI of the MI355X-native QSketch (two-level Multi-level
+ * Sub-Warp Sketch, "MSWS").
+ *
+ * The reference has no library boundary: its host API is the C++ virtual
+ * interface Sketch<Key_T, Count_T> (include/sketch/sketch.h:5-28) implemented
+ * by Sketch_GPU_Sub_Warp_Level (include/sketch/device_sketch/device_sketch_v3.h:
+ * 524-634), constructed with the default.h parameter set (lib/default.cu:29-63).
+ * Each entry point below replaces one member of that interface; the comment on
+ * each cites the member it replaces.
+ *
+ * Conventions (same as the reference unless stated):
+ *   - keys and counts are DEVICE pointers (the reference passes device buffers,
+ *     include/test/test.h:143-170);
+ *   - qs_insert / qs_query are blocking, like the reference's
+ *     cudaDeviceSynchronize after each launch (v3:592, v3:620);
+ *   - a handle is used by one host thread at a time (no internal locks, like
+ *     the reference);
+ *   - errors are negative return codes instead of the reference's undefined
+ *     behaviour (unchecked high pool, v3:119; release builds ignore errors,
+ *     include/qsketch.h:104-108).
+ *
+ * Semantics: every estimate equals the sequential stream-order restatement of
+ * v3 insert/search (v3:87-191, v3:310-408): the GPU insert is deterministic,
+ * including the allocation order of high buckets.
+ */
+#ifndef QSKETCH_H
+#define QSKETCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------- */
+#define QS_OK                 0
+#define QS_ERR_PARAM         -1  /* invalid parameter / argument             */
+#define QS_ERR_NOMEM         -2  /* device or host allocation failed         */
+#define QS_ERR_POOL          -3  /* high-bucket pool exhausted; state invalid until qs_clear */
+#define QS_ERR_HIP           -4  /* a HIP runtime call failed                */
+#define QS_ERR_STATE         -5  /* imported state is malformed              */
+#define QS_ERR_NODEVICE      -6  /* no usable GPU                            */
+
+/* ---- insert semantics of the allocation-triggering insert --------------- */
+#define QS_LITERAL 0 /* v3:187 as written: trigger insert adds to g[j mod 8] */
+#define QS_PAPER   1 /* doc/main.tex:433 intent: trigger insert adds to g[j]  */
+
+/* Parameter set.  Defaults are lib/default.cu:39-54 and the driver's
+ * constructor call src/main.cu:390-391:
+ *   Sketch_GPU_Sub_Warp_Level(n, 8, 1024, 32, 1, 1021).                     */
+typedef struct qs_params {
+    uint64_t n_low;      /* number of 32-byte low buckets (v3:544 _n_low)          */
+    uint32_t m_low;      /* words per low bucket: must be 8 (v3:544 _m_low)        */
+    uint32_t m_high;     /* words per high bucket: must be 32 (v3:545 _m_high)     */
+    uint64_t n_high;     /* high buckets incl. the unused bucket 0 (v3:545 _n_high)*/
+    uint32_t seed_sz;    /* hash seeds summed by hash_mul_add (lib/default.cu:52)  */
+    uint32_t n_masks;    /* mask table size, 1021 (lib/default.cu:54)             */
+    uint32_t mask_ones;  /* bits per mask: must be 3 (lib/default.cu:53)           */
+    uint32_t link_start; /* NEXT_LEVEL_ID_START: must be 1024 (lib/default.cu:45)  */
+    uint32_t low_limit;  /* promote when a selected byte exceeds it: must be 128 (v3:160) */
+    uint32_t key_bytes;  /* 4 (uint32 keys, src/main.cu:3) or 8 (uint64 keys)      */
+    int32_t  semantics;  /* QS_LITERAL (default) or QS_PAPER                      */
+    uint32_t hash_seed;  /* 0: draw the seeds from rng_seed; else use this value   */
+    uint64_t rng_seed;   /* replaces the reference's /dev/urandom engine seeding (tool.h:192-201) */
+    uint64_t chunk_keys; /* keys per internal insert chunk (0 = default 2^24)      */
+} qs_params;
+
+typedef struct qs_sketch qs_sketch;
+
+/* Fill *p with the default parameter set, n_low sized by the reference rule
+ * Sketch_GPU::number_of_buckets(expected_keys, 8, wlf)
+ * (device_sketch.h:371-376 + find_greatest_prime, lib/qsketch.cu:50-115).
+ * Replaces the driver's sizing call, src/main.cu:371-372. */
+int qs_default_params(qs_params *p, uint64_t expected_keys, double wlf);
+
+/* find_greatest_prime (lib/qsketch.cu:50-115). */
+uint64_t qs_find_greatest_prime(uint64_t upper_bound);
+
+/* Construct: allocate and zero both tables, draw the seeds and build the mask
+ * table.  Replaces the constructor Sketch_GPU_Sub_Warp_Level(...) (v3:544-563)
+ * and Sketch_GPU(...) (device_sketch.h:352-369).  Runs on the current HIP
+ * device. */
+int qs_create(const qs_params *p, qs_sketch **out);
+
+/* Free all device memory.  Replaces ~Sketch_GPU (device_sketch.h:387-394). */
+void qs_destroy(qs_sketch *s);
+
+/* Insert n keys (device pointer to uint32 or uint64, per key_bytes).
+ * Replaces virtual int Sketch::insert(Key_T*, size_t) (sketch.h:7; v3:570-599). */
+int qs_insert(qs_sketch *s, const void *d_keys, size_t n);
+
+/* Estimate the counts of n keys into d_counts (device, uint32[n]).
+ * Replaces virtual int Sketch::search(Key_T*, size_t, Count_T*) (sketch.h:8;
+ * v3:600-632). */
+int qs_query(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_counts);
+
+/* Zero both tables and reset the allocator.  Replaces virtual void
+ * Sketch::clear() (sketch.h:14; device_sketch.h:115-124, 378-381). */
+int qs_clear(qs_sketch *s);
+
+/* Stream-ordered variants (no synchronisation; stream is a hipStream_t or
+ * NULL for the null stream).  Build additions with no reference counterpart:
+ * the reference only has blocking calls on the default stream.  qs_insert_async
+ * still performs host-side chunking; errors are reported by the next blocking
+ * call or qs_sync. */
+int qs_insert_async(qs_sketch *s, const void *d_keys, size_t n, void *stream);
+int qs_query_async(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_counts,
+                   void *stream);
+int qs_sync(qs_sketch *s);
+
+/* ---- state import / export ---------------------------------------------- */
+/* Fills the reference layout: low  = uint32[n_low*8]  (word 7 = link),
+ * high = uint32[n_high*32], masks = uint32[n_masks] (little-endian bit j = byte
+ * counter j), next = the next high id.  Host pointers; any may be NULL.
+ * Fills the gap left by the declared-but-undefined read_file / write_file
+ * (include/qsketch.h:70-71). */
+int qs_export_state(qs_sketch *s, uint32_t *low, uint32_t *high, uint32_t *masks,
+                    uint32_t *seed, uint32_t *next);
+/* Replace the whole state (validated: link words in {0} or [1025, 1024+n_high),
+ * bytes <= 255, next consistent).  Host pointers, none may be NULL. */
+int qs_import_state(qs_sketch *s, const uint32_t *low, const uint32_t *high,
+                    const uint32_t *masks, uint32_t seed, uint32_t next);
+
+/* ---- statistics (build addition) ----------------------------------------- */
+typedef struct qs_stats {
+    uint64_t n_low, n_high, n_masks;
+    uint64_t bytes_low, bytes_high;     /* table sizes, cf. show_memory_usage
+                                           (device_sketch.h:403-407) */
+    uint64_t keys_inserted;
+    uint64_t chunks;                    /* internal insert chunks processed */
+    uint64_t chunks_with_fixup;         /* chunks that needed the ordered fix-up */
+    uint64_t fixup_keys;                /* keys replayed in stream order */
+    uint64_t linked_buckets;            /* high buckets in use (= next - 1025) */
+    uint32_t next_id;
+    uint32_t seed;
+} qs_stats;
+int qs_get_stats(qs_sketch *s, qs_stats *st);
+
+const char *qs_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSKETCH_H */

@@ -1,0 +1,330 @@
+"""MI355X-native QSketch: Python binding of the C ABI in include/qsketch.h.
+
+The compute path is the HIP library ``libqsketch_amd.so`` built in-tree by
+``qsketch_amd.build``.  Importing this package fails loudly when the library is
+missing; there is no CPU fallback.
+
+``Sketch_GPU_Sub_Warp_Level`` mirrors the reference's C++ class of the same
+name (include/sketch/device_sketch/device_sketch_v3.h:524-634) and its base
+interface ``Sketch<Key_T, Count_T>`` (include/sketch/sketch.h:5-28):
+``insert(keys, n)``, ``search(keys, n, counts)``, ``clear()``.  Keys and counts
+are device buffers (torch tensors on a HIP device, or raw device addresses).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqsketch_amd.so")
+
+QS_OK = 0
+QS_ERR_PARAM = -1
+QS_ERR_NOMEM = -2
+QS_ERR_POOL = -3
+QS_ERR_HIP = -4
+QS_ERR_STATE = -5
+QS_ERR_NODEVICE = -6
+QS_LITERAL = 0
+QS_PAPER = 1
+
+# lib/default.cu:39-54 and src/main.cu:390-391
+NEXT_LEVEL_ID_START = 1024
+HASH_MASK_ONES = 3
+HASH_MASK_TABLE_SZ = 1021
+SEED_SZ = 1
+
+
+class QSketchError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        self.code = code
+        msg = _lib().qs_strerror(code).decode() if _LIB is not None else str(code)
+        super().__init__(f"{what}: {msg} ({code})" if what else f"{msg} ({code})")
+
+
+class qs_params(C.Structure):
+    _fields_ = [
+        ("n_low", C.c_uint64), ("m_low", C.c_uint32), ("m_high", C.c_uint32),
+        ("n_high", C.c_uint64), ("seed_sz", C.c_uint32), ("n_masks", C.c_uint32),
+        ("mask_ones", C.c_uint32), ("link_start", C.c_uint32), ("low_limit", C.c_uint32),
+        ("key_bytes", C.c_uint32), ("semantics", C.c_int32), ("hash_seed", C.c_uint32),
+        ("rng_seed", C.c_uint64), ("chunk_keys", C.c_uint64),
+    ]
+
+
+class qs_stats(C.Structure):
+    _fields_ = [
+        ("n_low", C.c_uint64), ("n_high", C.c_uint64), ("n_masks", C.c_uint64),
+        ("bytes_low", C.c_uint64), ("bytes_high", C.c_uint64), ("keys_inserted", C.c_uint64),
+        ("chunks", C.c_uint64), ("chunks_with_fixup", C.c_uint64), ("fixup_keys", C.c_uint64),
+        ("linked_buckets", C.c_uint64), ("next_id", C.c_uint32), ("seed", C.c_uint32),
+    ]
+
+
+# every symbol declared in include/qsketch.h and include/qsketch_gen.h etc.
+_SIGS = {
+    "qs_default_params": (C.c_int, [C.POINTER(qs_params), C.c_uint64, C.c_double]),
+    "qs_find_greatest_prime": (C.c_uint64, [C.c_uint64]),
+    "qs_create": (C.c_int, [C.POINTER(qs_params), C.POINTER(C.c_void_p)]),
+    "qs_destroy": (None, [C.c_void_p]),
+    "qs_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "qs_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "qs_clear": (C.c_int, [C.c_void_p]),
+    "qs_insert_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "qs_query_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
+    "qs_sync": (C.c_int, [C.c_void_p]),
+    "qs_export_state": (C.c_int, [C.c_void_p] + [C.c_void_p] * 5),
+    "qs_import_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "qs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(qs_stats)]),
+    "qs_strerror": (C.c_char_p, [C.c_int]),
+    # generators (include/qsketch_gen.h)
+    "qs_zipf_create": (C.c_int, [C.c_double, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
+    "qs_zipf_destroy": (None, [C.c_void_p]),
+    "qs_gen_zipf_u32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "qs_gen_zipf_u32_host": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "qs_gen_zipf_rank_host": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "qs_gen_uniform_u32": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "qs_gen_uniform_u32_host": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "qs_rank_to_key32": (C.c_uint32, [C.c_uint64, C.c_uint32]),
+    "qs_gen_zipf_u64_noisy": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double,
+                                        C.c_void_p, C.c_void_p]),
+    "qs_gen_zipf_u64_noisy_host": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double,
+                                             C.c_void_p]),
+}
+
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m qsketch_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def lib():
+    """The loaded C ABI (ctypes.CDLL)."""
+    return _lib()
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != QS_OK:
+        raise QSketchError(rc, what)
+
+
+def _ptr(x) -> int:
+    """Device address of a torch tensor (contiguous) or an int address."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if not x.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        if not x.is_cuda:
+            raise ValueError("keys and counts must be device tensors")
+        return x.data_ptr()
+    raise TypeError(f"unsupported buffer {type(x)!r}")
+
+
+def default_params(expected_keys: int, wlf: float = 0.5) -> qs_params:
+    """Reference sizing rule (device_sketch.h:371-376, lib/qsketch.cu:50-115)."""
+    p = qs_params()
+    check(_lib().qs_default_params(C.byref(p), int(expected_keys), float(wlf)), "qs_default_params")
+    return p
+
+
+def find_greatest_prime(x: int) -> int:
+    return int(_lib().qs_find_greatest_prime(int(x)))
+
+
+def number_of_buckets(insert_keys_sz: int, m: int = 8, factor: float = 0.5) -> int:
+    """Sketch_GPU::number_of_buckets (device_sketch.h:371-376)."""
+    if m != 8:
+        row = int(((insert_keys_sz / factor) * HASH_MASK_ONES) / m)
+        return find_greatest_prime(row)
+    return int(default_params(insert_keys_sz, factor).n_low)
+
+
+class Sketch_GPU_Sub_Warp_Level:
+    """Two-level MSWS on the current HIP device.
+
+    Constructor arguments follow v3:544-547:
+    ``(n_low, m_low=8, n_high=1024, m_high=32, seed_sz=1, n_hash_mask_table=1021)``;
+    build additions are keyword-only.
+    """
+
+    def __init__(self, n_low: int, m_low: int = 8, n_high: int = 1024, m_high: int = 32,
+                 seed_sz: int = SEED_SZ, n_hash_mask_table: int = HASH_MASK_TABLE_SZ, *,
+                 key_bytes: int = 4, semantics: int = QS_LITERAL, rng_seed: int = 1,
+                 hash_seed: int = 0, chunk_keys: int = 0):
+        p = qs_params()
+        _lib().qs_default_params(C.byref(p), 1 << 20, 0.5)
+        p.n_low, p.m_low, p.n_high, p.m_high = int(n_low), int(m_low), int(n_high), int(m_high)
+        p.seed_sz, p.n_masks = int(seed_sz), int(n_hash_mask_table)
+        p.key_bytes, p.semantics, p.rng_seed = int(key_bytes), int(semantics), int(rng_seed)
+        p.hash_seed, p.chunk_keys = int(hash_seed) & 0xffffffff, int(chunk_keys)
+        self.params = p
+        self.key_bytes = int(key_bytes)
+        h = C.c_void_p()
+        check(_lib().qs_create(C.byref(p), C.byref(h)), "qs_create")
+        self._h = h
+
+    # -- Sketch<Key_T, Count_T> interface (sketch.h:7-14) ------------------
+    def insert(self, keys, keys_sz: int | None = None) -> int:
+        n = keys.numel() if keys_sz is None else int(keys_sz)
+        rc = _lib().qs_insert(self._h, _ptr(keys), n)
+        check(rc, "qs_insert")
+        return rc
+
+    def search(self, keys, keys_sz: int | None = None, counts=None):
+        n = keys.numel() if keys_sz is None else int(keys_sz)
+        if counts is None:
+            import torch
+            counts = torch.empty(n, dtype=torch.int32, device=keys.device)
+        check(_lib().qs_query(self._h, _ptr(keys), n, _ptr(counts)), "qs_query")
+        return counts
+
+    query = search
+
+    def clear(self) -> None:
+        check(_lib().qs_clear(self._h), "qs_clear")
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().qs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # -- build additions -----------------------------------------------------
+    def insert_async(self, keys, n: int, stream: int = 0) -> None:
+        check(_lib().qs_insert_async(self._h, _ptr(keys), int(n), C.c_void_p(stream)), "qs_insert_async")
+
+    def search_async(self, keys, n: int, counts, stream: int = 0) -> None:
+        check(_lib().qs_query_async(self._h, _ptr(keys), int(n), _ptr(counts), C.c_void_p(stream)),
+              "qs_query_async")
+
+    def sync(self) -> None:
+        check(_lib().qs_sync(self._h), "qs_sync")
+
+    def export_state(self):
+        import numpy as np
+        p = self.params
+        low = np.zeros(p.n_low * 8, dtype=np.uint32)
+        high = np.zeros(p.n_high * 32, dtype=np.uint32)
+        masks = np.zeros(p.n_masks, dtype=np.uint32)
+        seed = C.c_uint32()
+        nxt = C.c_uint32()
+        check(_lib().qs_export_state(self._h, low.ctypes.data, high.ctypes.data, masks.ctypes.data,
+                                     C.addressof(seed), C.addressof(nxt)), "qs_export_state")
+        return {"low": low, "high": high, "masks": masks, "seed": seed.value, "next": nxt.value}
+
+    def import_state(self, low, high, masks, seed: int, next_id: int) -> None:
+        import numpy as np
+        low = np.ascontiguousarray(low, dtype=np.uint32)
+        high = np.ascontiguousarray(high, dtype=np.uint32)
+        masks = np.ascontiguousarray(masks, dtype=np.uint32)
+        check(_lib().qs_import_state(self._h, low.ctypes.data, high.ctypes.data, masks.ctypes.data,
+                                     int(seed) & 0xffffffff, int(next_id)), "qs_import_state")
+
+    def stats(self) -> dict:
+        st = qs_stats()
+        check(_lib().qs_get_stats(self._h, C.byref(st)), "qs_get_stats")
+        return {f: getattr(st, f) for f, _ in qs_stats._fields_}
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def memory_bytes(self) -> int:
+        """Low + high table bytes (show_memory_usage, device_sketch.h:403-407)."""
+        return int(self.params.n_low) * 32 + int(self.params.n_high) * 128
+
+
+# convenience alias
+QSketch = Sketch_GPU_Sub_Warp_Level
+
+
+class Zipf:
+    """Alias table of Zipf(alpha) over ranks 1..universe (include/qsketch_gen.h)."""
+
+    def __init__(self, alpha: float, universe: int, device: bool = True):
+        h = C.c_void_p()
+        check(_lib().qs_zipf_create(float(alpha), int(universe), 1 if device else 0, C.byref(h)),
+              "qs_zipf_create")
+        self._h = h
+        self.alpha, self.universe = alpha, universe
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib().qs_zipf_destroy(self._h)
+            self._h = None
+
+    def keys_u32(self, seed: int, n: int, out=None, offset: int = 0, device: str = "cuda"):
+        import torch
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=device)
+        check(_lib().qs_gen_zipf_u32(self._h, seed, offset, n, _ptr(out), None), "qs_gen_zipf_u32")
+        return out
+
+    def keys_u32_host(self, seed: int, n: int, offset: int = 0):
+        import numpy as np
+        out = np.zeros(n, dtype=np.uint32)
+        check(_lib().qs_gen_zipf_u32_host(self._h, seed, offset, n, out.ctypes.data), "zipf host")
+        return out
+
+    def ranks_host(self, seed: int, n: int, offset: int = 0):
+        import numpy as np
+        out = np.zeros(n, dtype=np.uint32)
+        check(_lib().qs_gen_zipf_rank_host(self._h, seed, offset, n, out.ctypes.data), "zipf ranks")
+        return out
+
+    def keys_u64_noisy(self, seed: int, n: int, sigma: float = 1.5, out=None, offset: int = 0,
+                       device: str = "cuda"):
+        import torch
+        if out is None:
+            out = torch.empty(n, dtype=torch.int64, device=device)
+        check(_lib().qs_gen_zipf_u64_noisy(self._h, seed, offset, n, float(sigma), _ptr(out), None),
+              "qs_gen_zipf_u64_noisy")
+        return out
+
+    def keys_u64_noisy_host(self, seed: int, n: int, sigma: float = 1.5, offset: int = 0):
+        import numpy as np
+        out = np.zeros(n, dtype=np.uint64)
+        check(_lib().qs_gen_zipf_u64_noisy_host(self._h, seed, offset, n, float(sigma), out.ctypes.data),
+              "zipf u64 host")
+        return out
+
+
+def uniform_u32(seed: int, n: int, out=None, offset: int = 0, device: str = "cuda"):
+    import torch
+    if out is None:
+        out = torch.empty(n, dtype=torch.int32, device=device)
+    check(_lib().qs_gen_uniform_u32(seed, offset, n, _ptr(out), None), "qs_gen_uniform_u32")
+    return out
+
+
+def uniform_u32_host(seed: int, n: int, offset: int = 0):
+    import numpy as np
+    out = np.zeros(n, dtype=np.uint32)
+    check(_lib().qs_gen_uniform_u32_host(seed, offset, n, out.ctypes.data), "uniform host")
+    return out
+
+
+def rank_to_key32(seed: int, rank: int) -> int:
+    return int(_lib().qs_rank_to_key32(seed, rank))
+
