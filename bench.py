@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""QSketch on one MI355X: insert + query throughput at Zipf alpha = 1.1.
+
+Workload (one "step"): clear the sketch, insert a stream of 2^28 uint32 keys
+drawn Zipf(1.1) over a 2^26-key universe (rank -> key by a seeded permutation),
+then query every distinct key of the stream.  The sketch uses the reference
+sizing rule at work-load factor 0.5 (n_low = 201,326,557 buckets = 6 GiB) and a
+2^18-bucket high pool.  Keys are generated on the GPU before the timed region.
+
+Prints one JSON line.  value = (inserted + queried keys) / wall time of the K
+timed steps, in G keys/s.  The roofline object is for the optimistic insert
+kernel (the dominant one): algorithmic bytes 68 B per key (4 B key + 32 B bucket
+read + 32 B bucket write), divided by its summed launch durations measured with
+HIP events on its launch stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+INSERT_BYTES_PER_KEY = 68   # u32 key + 32 B bucket read + 32 B bucket write
+QUERY_BYTES_PER_KEY = 40    # u32 key + 32 B bucket read + 4 B estimate
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--log2n", type=int, default=28, help="log2 of the stream length")
+    ap.add_argument("--log2u", type=int, default=26, help="log2 of the key universe")
+    ap.add_argument("--alpha", type=float, default=1.1, help="Zipf exponent; 0 = uniform keys")
+    ap.add_argument("--wlf", type=float, default=0.5, help="work-load factor of the sizing rule")
+    ap.add_argument("--n-high", type=int, default=1 << 18)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-log2", type=int, default=24,
+                    help="log2 of the stream prefix the CPU baseline inserts")
+    return ap.parse_args()
+
+
+def cpu_baseline(keys_dev, n_low, n_high, sample_log2):
+    """Time the oracle (the restated sequential semantics) on host cores over a
+    bounded prefix of the same stream: insert the prefix, query its distinct keys."""
+    import numpy as np
+    from oracle import pyoracle as po
+    n = min(1 << sample_log2, keys_dev.numel())
+    keys = keys_dev[:n].cpu().numpy().view(np.uint32).copy()
+    uniq = np.unique(keys)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    o = po.OracleSketch(n_low, n_high, 1021, rng_seed=1)
+    o.clear()  # touch every page before timing
+    t0 = time.perf_counter()
+    rc = o.insert(keys, nthreads=threads)
+    t1 = time.perf_counter()
+    o.query(uniq, nthreads=threads)
+    t2 = time.perf_counter()
+    if rc != 0:
+        raise RuntimeError(f"oracle insert failed: {rc}")
+    return {
+        "value": round((n + uniq.size) / (t2 - t0) / 1e9, 5),
+        "unit": "Gkeys/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first 2^{sample_log2} keys of the same stream inserted ({t1 - t0:.2f} s, "
+                  f"{n / (t1 - t0) / 1e6:.1f} Mkeys/s), then its {uniq.size} distinct keys queried "
+                  f"({t2 - t1:.2f} s); bucket-range partitioned oracle on {threads} threads",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
+                                   "designed (see DESIGN.md) but not built", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    import qsketch_amd as q
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    N = 1 << a.log2n
+    U = 1 << a.log2u
+    if a.alpha > 0:
+        z = q.Zipf(a.alpha, U)
+        keys = z.keys_u32(seed=1, n=N)
+        dist = f"zipf{a.alpha}"
+    else:
+        keys = q.uniform_u32(seed=1, n=N)
+        dist = "uniform"
+    uniq = torch.unique(keys)
+    counts = torch.empty_like(uniq)
+    n_low = q.number_of_buckets(N, 8, a.wlf)
+    sk = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, a.n_high, 32, 1, 1021, rng_seed=1)
+    torch.cuda.synchronize()
+
+    def step():
+        sk.clear()
+        sk.insert(keys, N)
+        sk.search(uniq, uniq.numel(), counts)
+
+    for _ in range(a.warmup):
+        step()
+    sk.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tim = sk.timing()
+    st = sk.stats()
+    sk.set_timing(False)
+
+    dt = t1 - t0
+    n_q = uniq.numel()
+    value = a.steps * (N + n_q) / dt / 1e9
+    ins_keys = tim["insert_keys"]
+    ins_ms = tim["insert_kernel_ms"]
+    achieved = ins_keys * INSERT_BYTES_PER_KEY / (ins_ms * 1e-3) / 1e9 if ins_ms > 0 else 0.0
+    q_ms = tim["query_kernel_ms"]
+    q_achieved = tim["query_keys"] * QUERY_BYTES_PER_KEY / (q_ms * 1e-3) / 1e9 if q_ms > 0 else 0.0
+    kern = {k: v for k, v in tim["kernels"].items() if k != "query" and v["launches"]}
+    dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
+    out = {
+        "metric": "insert & query Gkeys/s (% HBM roofline) at Zipf 1.1; ARE at equal memory",
+        "value": round(value, 4),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(dt / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain QSketch insert 2^{a.log2n} u32 keys {dist} over 2^{a.log2u} keys, "
+                        f"then query all {n_q} distinct keys",
+            "n_low": n_low, "n_high": a.n_high, "wlf": a.wlf, "keys": N, "distinct": n_q,
+        },
+        "insert_gkeys_s": round(ins_keys / (ins_ms * 1e-3) / 1e9, 3) if ins_ms > 0 else None,
+        "query_gkeys_s": round(tim["query_keys"] / (q_ms * 1e-3) / 1e9, 3) if q_ms > 0 else None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "insert path: k_bsort_up/k_sort_scan_digit/k_bsort_down per radix pass, "
+                      "k_seg, k_ordered, k_assign, k_tail",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_key": INSERT_BYTES_PER_KEY,
+            "keys_per_launch": round(ins_keys / max(1, kern.get("seg", {"launches": 1})["launches"])),
+            "path_ms_per_step": round(ins_ms / a.steps, 4),
+            "dominant_kernel": dom,
+            "kernel_ms_per_step": {k: round(v["ms"] / a.steps, 4) for k, v in kern.items()},
+        },
+        "roofline_query": {
+            "bound": "hbm", "kernel": "k_query<4>", "achieved": round(q_achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(q_achieved / HBM_PEAK_GBS, 4),
+            "bytes_per_key": QUERY_BYTES_PER_KEY,
+            "avg_launch_ms": round(q_ms / max(1, tim["query_kernel_launches"]), 4),
+        },
+        "fixup": {"segments_replayed_per_step": st["fixup_keys"] // max(1, a.steps + a.warmup),
+                  "linked_buckets": st["linked_buckets"]},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(keys, n_low, a.n_high, a.cpu_sample_log2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -138,6 +138,34 @@ typedef struct qs_stats {
 } qs_stats;
 int qs_get_stats(qs_sketch *s, qs_stats *st);
 
+/* ---- kernel timing (build addition) --------------------------------------
+ * When enabled, the library records HIP events on the launch stream around
+ * every kernel launch of the insert path and of the query (the reference only
+ * timed whole blocking calls on the host, include/tool/tool.h:26-50).
+ * Enabling resets the totals; qs_get_timing synchronises and resolves them.
+ * kernel_ms[k] / kernel_launches[k] per kernel group k:                      */
+#define QS_TK_SORT_UP   0  /* radix sort by bucket: digit histograms         */
+#define QS_TK_SORT_SCAN 1  /* radix sort: scan of the histograms             */
+#define QS_TK_SORT_DOWN 2  /* radix sort: stable scatter                     */
+#define QS_TK_SEG       3  /* per-bucket-segment apply (the table sweep)     */
+#define QS_TK_ORDERED   4  /* in-order replay of segments near the limit     */
+#define QS_TK_ASSIGN    5  /* allocation ids in stream order                 */
+#define QS_TK_TAIL      6  /* long (hot) segments: windowed high adds        */
+#define QS_TK_QUERY     7  /* query kernel                                   */
+#define QS_TIMING_KERNELS 8
+typedef struct qs_timing {
+    double   insert_kernel_ms;      /* sum of kernel_ms[0..6] */
+    uint64_t insert_kernel_launches;
+    uint64_t insert_keys;           /* keys inserted while timing */
+    double   query_kernel_ms;       /* = kernel_ms[QS_TK_QUERY] */
+    uint64_t query_kernel_launches;
+    uint64_t query_keys;
+    double   kernel_ms[QS_TIMING_KERNELS];
+    uint64_t kernel_launches[QS_TIMING_KERNELS];
+} qs_timing;
+int qs_set_timing(qs_sketch *s, int enable);
+int qs_get_timing(qs_sketch *s, qs_timing *t);
+
 const char *qs_strerror(int code);
 
 #ifdef __cplusplus

@@ -61,6 +61,18 @@ class qs_stats(C.Structure):
     ]
 
 
+TIMING_KERNELS = ["sort_up", "sort_scan", "sort_down", "seg", "ordered", "assign", "tail", "query"]
+
+
+class qs_timing(C.Structure):
+    _fields_ = [
+        ("insert_kernel_ms", C.c_double), ("insert_kernel_launches", C.c_uint64),
+        ("insert_keys", C.c_uint64), ("query_kernel_ms", C.c_double),
+        ("query_kernel_launches", C.c_uint64), ("query_keys", C.c_uint64),
+        ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8),
+    ]
+
+
 # every symbol declared in include/qsketch.h and include/qsketch_gen.h etc.
 _SIGS = {
     "qs_default_params": (C.c_int, [C.POINTER(qs_params), C.c_uint64, C.c_double]),
@@ -77,6 +89,8 @@ _SIGS = {
     "qs_import_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "qs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(qs_stats)]),
     "qs_strerror": (C.c_char_p, [C.c_int]),
+    "qs_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "qs_get_timing": (C.c_int, [C.c_void_p, C.POINTER(qs_timing)]),
     # generators (include/qsketch_gen.h)
     "qs_zipf_create": (C.c_int, [C.c_double, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
     "qs_zipf_destroy": (None, [C.c_void_p]),
@@ -245,6 +259,18 @@ class Sketch_GPU_Sub_Warp_Level:
         st = qs_stats()
         check(_lib().qs_get_stats(self._h, C.byref(st)), "qs_get_stats")
         return {f: getattr(st, f) for f, _ in qs_stats._fields_}
+
+    def set_timing(self, enable: bool = True) -> None:
+        """Record HIP events around every insert/query kernel launch (resets totals)."""
+        check(_lib().qs_set_timing(self._h, 1 if enable else 0), "qs_set_timing")
+
+    def timing(self) -> dict:
+        t = qs_timing()
+        check(_lib().qs_get_timing(self._h, C.byref(t)), "qs_get_timing")
+        d = {f: getattr(t, f) for f, _ in qs_timing._fields_[:6]}
+        d["kernels"] = {k: {"ms": t.kernel_ms[i], "launches": t.kernel_launches[i]}
+                        for i, k in enumerate(TIMING_KERNELS)}
+        return d
 
     @property
     def handle(self) -> int:

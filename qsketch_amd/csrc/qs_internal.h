@@ -101,19 +101,18 @@ __device__ static inline void qs_hash_index(uint64_t key, const qs_hashpar &hp, 
 // Small device-resident control block of one sketch.
 struct qs_dev_state {
     uint32_t next;      // next high id (starts at 1025)
-    uint32_t any_viol;  // set by the optimistic insert when a byte would pass the limit
-    uint32_t r_n;       // number of keys routed to the ordered fix-up
-    uint32_t n_heads;   // bucket segments in the fix-up
+    uint32_t n_ord;     // bucket segments queued for the in-order replay
+    uint32_t n_long;    // long bucket segments (handled by the windowed tail kernel)
     uint32_t n_trig;    // allocations triggered in the current chunk
-    uint32_t err;       // sticky error: 1 = pool exhausted, 2 = trigger overflow
-    uint32_t pad[2];
+    uint32_t err;       // sticky error: 1 = pool exhausted
+    uint32_t pad[3];
 };
 
-// One allocation found by the ordered fix-up (filled before ids are known).
+// One allocation found by the in-order replay (filled before ids are known).
 struct qs_trig {
     uint32_t idx;        // chunk-local stream index of the triggering insert
     uint32_t bucket;     // low bucket
-    uint32_t tail_start; // first fix-up position handled by the tail kernel
+    uint32_t tail_start; // first sorted position handled by the tail kernel
     uint32_t pad;
     uint32_t hc[32];     // high counters accumulated up to tail_start (incl. trigger)
 };
@@ -125,26 +124,29 @@ struct qs_sketch {
     uint32_t seed;
     uint32_t *h_masks;   // host copy
     qs_hashpar hp;
-    uint32_t chunk;      // max keys per chunk
-    uint32_t tile_n;     // number of rollback tiles per chunk
+    uint64_t chunk;      // capacity (keys) of the chunk scratch
+    int bucket_bits;     // bits of (n_low - 1)
 
     uint32_t *d_low;
     uint32_t *d_high;
     uint32_t *d_masks;
     qs_dev_state *d_st;
-    uint32_t *d_viol;       // bitmap, n_low bits
-    uint32_t *d_tile_cnt;   // per rollback tile
-    uint32_t *d_tile_off;
-    uint32_t *d_rb[2];      // fix-up buckets (double buffer for the sort)
-    uint32_t *d_ri[2];      // fix-up stream indices
-    uint32_t *d_hist;       // radix sort histograms
-    uint32_t *d_heads;
+    uint32_t *d_longbm;     // bitmap over buckets: segment handled by the tail kernel
+    void *d_k[2];           // chunk keys sorted by bucket (double buffer)
+    uint32_t *d_i[2];       // their chunk-local stream indices
+    uint32_t *d_hist;       // radix histograms (shared by both sorts)
+    uint32_t *d_ord;        // heads of segments for the in-order replay
+    uint32_t *d_long;       // heads of long segments
     qs_trig *d_trig;
     uint32_t trig_cap;
     uint32_t *d_tk[2];      // trigger sort keys / values
     uint32_t *d_tv[2];
-    uint32_t *d_tail_of;    // n_high: fix-up tail start per high bucket
+    uint32_t *d_tail_of;    // n_high: tail start per high bucket (current chunk)
     qs_dev_state *h_st;     // pinned host mirror
+
+    int timing;             // record events around kernel launches
+    void *tim;              // event lists (qs_sketch.hip)
+    qs_timing tot;
 
     uint64_t keys_inserted;
     uint64_t chunks;
@@ -159,3 +161,46 @@ struct qs_sketch {
 int qs_sort_pairs(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist, uint32_t count,
                   int key_bits, hipStream_t stream);
 size_t qs_sort_hist_words(uint32_t max_count);
+// Exclusive scan of each row hist[d][0..ntiles) for d < nbins; totals[d] = row sum.
+void qs_scan_digits(uint32_t *hist, uint32_t ntiles, uint32_t nbins, uint32_t *totals,
+                    hipStream_t stream);
+
+// insert pipeline (qs_insert.hip)
+int qs_insert_chunks(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream);
+int qs_ensure_scratch(qs_sketch *s, uint64_t chunk);
+void qs_free_scratch(qs_sketch *s);
+uint64_t qs_scratch_hist_words(uint64_t chunk);
+
+// timing hooks (qs_sketch.hip)
+struct qs_evpair {
+    hipEvent_t a, b;
+    int kind; // QS_TK_* kernel group, or QS_TIMING_KERNELS for a chunk marker
+    uint64_t keys;
+};
+void qs_tim_begin(qs_sketch *s, qs_evpair &e, int kind, uint64_t keys, hipStream_t st);
+void qs_tim_end(qs_sketch *s, qs_evpair &e, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// shared device helpers
+// ---------------------------------------------------------------------------
+__device__ static inline uint32_t qs_bucket(uint32_t key, const qs_hashpar &hp)
+{
+    return qs_fastmod(key * hp.seed, hp.low);
+}
+
+__device__ static inline uint32_t qs_bucket(uint64_t key, const qs_hashpar &hp)
+{
+    return (uint32_t)((key * (uint64_t)hp.seed) % (uint64_t)hp.n_low);
+}
+
+__device__ static inline uint32_t qs_maskidx(uint32_t key, const qs_hashpar &hp)
+{
+    return qs_fastmod(key * hp.seed, hp.masks);
+}
+
+__device__ static inline uint32_t qs_maskidx(uint64_t key, const qs_hashpar &hp)
+{
+    return (uint32_t)((key * (uint64_t)hp.seed) % (uint64_t)hp.n_masks);
+}
+
+template <int KB> using KeyT = typename qs_key<KB>::T;

@@ -5,23 +5,10 @@
 // class Sketch_GPU_Sub_Warp_Level v3:524-634) and its parameter / mask / seed
 // set-up (lib/default.cu, lib/hashmask.cu, include/tool/tool.h:186-220).
 //
-// Insert is deterministic: its result equals the sequential stream-order
-// semantics of v3 (see the oracle) including the order in which high buckets
-// are allocated.  Per chunk of keys:
-//   1. k_insert (optimistic, one lane per key): linked buckets -> three u32
-//      atomics on the high bucket (commutative); unlinked buckets -> returning
-//      packed-byte atomics on the low words; a lane that sees a selected byte
-//      >= 129 before its add flags the bucket in a bitmap.
-//      No flag on a bucket  <=>  no selected byte passed 129 in any order
-//      <=>  no allocation would happen for it in stream order, so the
-//      commutative result is exact.
-//   2. only if some bucket was flagged: every insert of a flagged bucket is
-//      subtracted again (exact mod 2^32, so even a wrapped byte is restored),
-//      compacted in stream order, radix-sorted by bucket (stable), and each
-//      bucket's segment is replayed in stream order by one wavefront
-//      (k_ordered) up to its allocation point; the rest of a long segment is
-//      added to the high bucket in parallel (k_tail).  Allocations of a chunk
-//      get ids in stream order of their triggering insert (k_assign).
+// The insert pipeline lives in qs_insert.hip; this file holds the host API
+// (parameters, seeds and masks with the reference's libstdc++ call pattern,
+// create / destroy / clear / import / export / stats / timing) and the query
+// kernel.
 #include "qs_internal.h"
 
 #include <climits>
@@ -29,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <vector>
 
 #define CK(x)                                                                                    \
     do {                                                                                         \
@@ -39,9 +27,60 @@
     } while (0)
 
 #define INS_THREADS 256
-#define RB_THREADS 256
-#define RB_ITEMS 8
-#define RB_TILE (RB_THREADS * RB_ITEMS)
+
+// Pending event pairs of the timing hooks, resolved by qs_get_timing.
+struct qs_timer {
+    std::vector<qs_evpair> pend;
+};
+
+void qs_tim_begin(qs_sketch *s, qs_evpair &e, int kind, uint64_t keys, hipStream_t st)
+{
+    e.kind = kind;
+    e.keys = keys;
+    e.a = e.b = nullptr;
+    if (!s->timing)
+        return;
+    hipEventCreate(&e.a);
+    hipEventCreate(&e.b);
+    hipEventRecord(e.a, st);
+}
+
+void qs_tim_end(qs_sketch *s, qs_evpair &e, hipStream_t st)
+{
+    if (!s->timing || !e.a)
+        return;
+    hipEventRecord(e.b, st);
+    ((qs_timer *)s->tim)->pend.push_back(e);
+}
+
+static void tim_resolve(qs_sketch *s)
+{
+    if (!s->tim)
+        return;
+    qs_timer *t = (qs_timer *)s->tim;
+    for (auto &e : t->pend) {
+        float ms = 0.f;
+        hipEventSynchronize(e.b);
+        hipEventElapsedTime(&ms, e.a, e.b);
+        if (e.kind >= 0 && e.kind < QS_TIMING_KERNELS) {
+            s->tot.kernel_ms[e.kind] += ms;
+            s->tot.kernel_launches[e.kind]++;
+            if (e.kind == QS_TK_QUERY) {
+                s->tot.query_kernel_ms += ms;
+                s->tot.query_kernel_launches++;
+                s->tot.query_keys += e.keys;
+            } else {
+                s->tot.insert_kernel_ms += ms;
+                s->tot.insert_kernel_launches++;
+            }
+        } else if (e.kind == QS_TIMING_KERNELS) {
+            s->tot.insert_keys += e.keys; // marker: keys of one insert chunk
+        }
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    t->pend.clear();
+}
 
 // ===========================================================================
 // Host: parameter set, prime table, seeds and masks
@@ -224,7 +263,6 @@ __device__ static inline uint32_t lds_mask(const uint32_t *s_masks, uint32_t mu)
     return s_masks[mu];
 }
 
-template <int KB> using KeyT = typename qs_key<KB>::T;
 
 // ===========================================================================
 // Query: one lane per key, the 32-byte bucket as two 16-byte loads.
@@ -269,490 +307,10 @@ __global__ __launch_bounds__(INS_THREADS) void k_query(const KeyT<KB> *__restric
 }
 
 // ===========================================================================
-// Insert step 1: optimistic commutative insert.
-// ===========================================================================
-template <int KB>
-__global__ __launch_bounds__(INS_THREADS) void k_insert(const KeyT<KB> *__restrict__ keys, uint32_t n,
-                                                        uint32_t *__restrict__ low,
-                                                        uint32_t *__restrict__ high,
-                                                        const uint32_t *__restrict__ masks,
-                                                        qs_hashpar hp, uint32_t *__restrict__ viol,
-                                                        qs_dev_state *__restrict__ st)
-{
-    __shared__ uint32_t s_masks[QS_MAX_MASKS];
-    for (uint32_t i = threadIdx.x; i < hp.n_masks; i += INS_THREADS)
-        s_masks[i] = masks[i];
-    __syncthreads();
-    const uint32_t stride = gridDim.x * INS_THREADS;
-    for (uint32_t i = blockIdx.x * INS_THREADS + threadIdx.x; i < n; i += stride) {
-        uint32_t b, mu;
-        qs_hash_index(keys[i], hp, b, mu);
-        uint32_t M = lds_mask(s_masks, mu);
-        uint32_t *bk = low + (size_t)b * 8u;
-        uint32_t w = bk[7];
-        if (w > QS_LINK_START) {
-            uint32_t *g = high + (size_t)(w - QS_LINK_START) * 32u;
-            uint32_t m = M;
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                atomicAdd(g + __builtin_ctz(m), 1u);
-                m &= m - 1u;
-            }
-        } else {
-            low_adds la = split_mask(M);
-            bool v = false;
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                if (la.a[r]) {
-                    uint32_t old = atomicAdd(bk + la.k[r], la.a[r]);
-                    v |= over_limit(old, la.a[r]);
-                }
-            }
-            if (v) {
-                atomicOr(viol + (b >> 5), 1u << (b & 31u));
-                __hip_atomic_store(&st->any_viol, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-    }
-}
-
-// ===========================================================================
-// Insert step 2a: undo the optimistic adds of every unlinked insert whose
-// bucket is flagged, and count them per tile of RB_TILE consecutive keys.
-// ===========================================================================
-template <int KB>
-__global__ __launch_bounds__(RB_THREADS) void k_rollback(const KeyT<KB> *__restrict__ keys, uint32_t n,
-                                                         uint32_t *__restrict__ low,
-                                                         const uint32_t *__restrict__ masks,
-                                                         qs_hashpar hp,
-                                                         const uint32_t *__restrict__ viol,
-                                                         uint32_t *__restrict__ tile_cnt)
-{
-    __shared__ uint32_t s_w[RB_THREADS / QS_WAVE];
-    const uint32_t t = threadIdx.x;
-    const uint32_t base = blockIdx.x * RB_TILE;
-    uint32_t cnt = 0;
-#pragma unroll
-    for (int r = 0; r < RB_ITEMS; ++r) {
-        uint32_t i = base + r * RB_THREADS + t;
-        if (i < n) {
-            uint32_t b, mu;
-            qs_hash_index(keys[i], hp, b, mu);
-            if (bit_test(viol, b)) {
-                uint32_t *bk = low + (size_t)b * 8u;
-                if (bk[7] <= QS_LINK_START) {
-                    low_adds la = split_mask(masks[mu]);
-#pragma unroll
-                    for (int q = 0; q < 3; ++q)
-                        if (la.a[q])
-                            atomicSub(bk + la.k[q], la.a[q]);
-                    ++cnt;
-                }
-            }
-        }
-    }
-    // block sum
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        cnt += __shfl_xor(cnt, o, 64);
-    if ((t & 63u) == 0)
-        s_w[t >> 6] = cnt;
-    __syncthreads();
-    if (t == 0) {
-        uint32_t s = 0;
-        for (int k = 0; k < RB_THREADS / QS_WAVE; ++k)
-            s += s_w[k];
-        tile_cnt[blockIdx.x] = s;
-    }
-}
-
-// Exclusive scan of n values (single workgroup); total to *total.
-__global__ __launch_bounds__(1024) void k_scan_excl(const uint32_t *__restrict__ in,
-                                                    uint32_t *__restrict__ out, uint32_t n,
-                                                    uint32_t *__restrict__ total)
-{
-    __shared__ uint32_t s_w[16];
-    __shared__ uint32_t s_carry;
-    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
-    if (t == 0)
-        s_carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < n; base += 1024) {
-        uint32_t i = base + t;
-        uint32_t v = i < n ? in[i] : 0u;
-        uint32_t x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= (uint32_t)o)
-                x += y;
-        }
-        if (lane == 63)
-            s_w[wv] = x;
-        __syncthreads();
-        if (t < 16) {
-            uint32_t a = s_w[t], sc = a;
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) {
-                uint32_t y = __shfl_up(sc, o, 16);
-                if ((t & 15u) >= (uint32_t)o)
-                    sc += y;
-            }
-            s_w[t] = sc - a;
-        }
-        __syncthreads();
-        uint32_t carry = s_carry;
-        if (i < n)
-            out[i] = carry + s_w[wv] + x - v;
-        __syncthreads();
-        if (t == 1023)
-            s_carry = carry + s_w[wv] + x;
-        __syncthreads();
-    }
-    if (t == 0)
-        *total = s_carry;
-}
-
-// Insert step 2b: write the flagged inserts (bucket, stream index) in stream
-// order at their tile's offset.
-template <int KB>
-__global__ __launch_bounds__(RB_THREADS) void k_compact(const KeyT<KB> *__restrict__ keys, uint32_t n,
-                                                        const uint32_t *__restrict__ low,
-                                                        qs_hashpar hp,
-                                                        const uint32_t *__restrict__ viol,
-                                                        const uint32_t *__restrict__ tile_cnt,
-                                                        const uint32_t *__restrict__ tile_off,
-                                                        uint32_t *__restrict__ rb,
-                                                        uint32_t *__restrict__ ri)
-{
-    __shared__ uint32_t s_w[RB_THREADS / QS_WAVE];
-    if (tile_cnt[blockIdx.x] == 0)
-        return;
-    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
-    const uint64_t lt = (1ull << lane) - 1ull;
-    const uint32_t base = blockIdx.x * RB_TILE;
-    uint32_t run = tile_off[blockIdx.x];
-    for (int r = 0; r < RB_ITEMS; ++r) {
-        uint32_t i = base + r * RB_THREADS + t;
-        bool f = false;
-        uint32_t b = 0, mu;
-        if (i < n) {
-            qs_hash_index(keys[i], hp, b, mu);
-            f = bit_test(viol, b) && low[(size_t)b * 8u + 7u] <= QS_LINK_START;
-        }
-        uint64_t bal = __ballot(f);
-        if (lane == 0)
-            s_w[wv] = (uint32_t)__popcll(bal);
-        __syncthreads();
-        uint32_t off = run, tot = 0;
-        for (uint32_t k = 0; k < RB_THREADS / QS_WAVE; ++k) {
-            uint32_t c = s_w[k];
-            if (k < wv)
-                off += c;
-            tot += c;
-        }
-        if (f) {
-            uint32_t pos = off + (uint32_t)__popcll(bal & lt);
-            rb[pos] = b;
-            ri[pos] = i;
-        }
-        run += tot;
-        __syncthreads();
-    }
-}
-
-__global__ void k_heads(const uint32_t *__restrict__ rb, uint32_t r_n, uint32_t *__restrict__ heads,
-                        qs_dev_state *__restrict__ st)
-{
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < r_n && (p == 0 || rb[p] != rb[p - 1])) {
-        uint32_t k = atomicAdd(&st->n_heads, 1u);
-        heads[k] = p;
-    }
-}
-
-__device__ static inline uint32_t kth_set_bit(uint64_t m, uint32_t k)
-{
-    // position of the k-th (0-based) set bit of m; caller guarantees popc(m) > k
-    uint32_t pos = 0;
-#pragma unroll
-    for (int w = 32; w >= 1; w >>= 1) {
-        uint64_t lowm = (w == 64) ? ~0ull : ((1ull << w) - 1ull);
-        uint32_t c = (uint32_t)__popcll(m & lowm);
-        if (k >= c) {
-            k -= c;
-            m >>= w;
-            pos += (uint32_t)w;
-        }
-    }
-    return pos;
-}
-
-// Insert step 3: replay each flagged bucket's segment in stream order on one
-// wavefront until its allocation point (at most ceil(28*129/3) inserts can
-// precede it).  Lane j (< 28) holds byte counter j; a batch of 64 inserts is
-// transposed into 28 ballots B_j, and the first insert that would see a
-// selected byte > 128 is the (129 - c_j)-th set bit of B_j, minimised over j.
-template <int KB>
-__global__ __launch_bounds__(256) void k_ordered(const KeyT<KB> *__restrict__ keys,
-                                                 const uint32_t *__restrict__ rb,
-                                                 const uint32_t *__restrict__ ri, uint32_t r_n,
-                                                 const uint32_t *__restrict__ heads,
-                                                 uint32_t *__restrict__ low,
-                                                 const uint32_t *__restrict__ masks, qs_hashpar hp,
-                                                 uint32_t *__restrict__ viol,
-                                                 qs_trig *__restrict__ trig, uint32_t trig_cap,
-                                                 qs_dev_state *__restrict__ st, int paper)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t gw = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
-    const uint32_t nw = gridDim.x * (blockDim.x / 64u);
-    const uint32_t n_heads = st->n_heads;
-    for (uint32_t h = gw; h < n_heads; h += nw) {
-        const uint32_t p0 = heads[h];
-        const uint32_t b = rb[p0];
-        uint32_t *bk = low + (size_t)b * 8u;
-        uint32_t word = lane < 7u ? bk[lane] : 0u;
-        uint32_t c = __shfl(word, (int)(lane >> 2), 64);
-        c = lane < 28u ? ((c >> ((lane & 3u) * 8u)) & 0xffu) : 0u;
-        uint32_t hc = 0;
-        bool fired = false;
-        uint32_t t_idx = 0, tail_start = 0;
-        for (uint32_t p = p0;; p += 64u) {
-            uint32_t q = p + lane;
-            bool valid = q < r_n && rb[q] == b;
-            uint64_t vm = __ballot(valid);
-            if (vm == 0)
-                break;
-            uint32_t idx = 0, M = 0;
-            if (valid) {
-                uint32_t bb, mu;
-                idx = ri[q];
-                qs_hash_index(keys[idx], hp, bb, mu);
-                M = masks[mu];
-            }
-            uint64_t Bj = 0;
-#pragma unroll
-            for (int j = 0; j < 28; ++j) {
-                uint64_t bal = __ballot((M >> j) & 1u);
-                Bj = (lane == (uint32_t)j) ? bal : Bj;
-            }
-            uint32_t need = c > QS_LOW_LIMIT ? 0u : (QS_LOW_LIMIT + 1u - c);
-            uint32_t pos = 64u;
-            if (lane < 28u && (uint32_t)__popcll(Bj) > need)
-                pos = kth_set_bit(Bj, need);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1)
-                pos = min(pos, (uint32_t)__shfl_xor(pos, o, 64));
-            if (pos == 64u) {
-                c += (uint32_t)__popcll(Bj);
-            } else {
-                uint64_t before = pos == 0u ? 0ull : ((1ull << pos) - 1ull);
-                uint64_t after = pos >= 63u ? 0ull : (~0ull << (pos + 1u));
-                c += (uint32_t)__popcll(Bj & before);
-                uint32_t Mt = __shfl(M, (int)pos, 64);
-                t_idx = __shfl(idx, (int)pos, 64);
-                if (paper)
-                    hc += lane < 32u ? ((Mt >> lane) & 1u) : 0u;
-                else
-                    hc += lane < 8u ? (uint32_t)__popc(Mt & (0x01010101u << lane)) : 0u;
-                hc += (uint32_t)__popcll(Bj & after);
-                fired = true;
-                tail_start = p + 64u;
-                break;
-            }
-            if (vm != ~0ull)
-                break;
-        }
-        // write the low counters back (bucket is owned by this wavefront)
-        uint32_t w4 = (uint32_t)__shfl(c, (int)(4u * lane) & 63, 64) |
-                      ((uint32_t)__shfl(c, (int)(4u * lane + 1u) & 63, 64) << 8) |
-                      ((uint32_t)__shfl(c, (int)(4u * lane + 2u) & 63, 64) << 16) |
-                      ((uint32_t)__shfl(c, (int)(4u * lane + 3u) & 63, 64) << 24);
-        if (lane < 7u)
-            bk[lane] = w4;
-        if (lane == 0)
-            atomicAnd(viol + (b >> 5), ~(1u << (b & 31u)));
-        if (fired) {
-            uint32_t slot = 0;
-            if (lane == 0)
-                slot = atomicAdd(&st->n_trig, 1u);
-            slot = __shfl(slot, 0, 64);
-            if (slot < trig_cap) {
-                qs_trig *tr = trig + slot;
-                if (lane == 0) {
-                    tr->idx = t_idx;
-                    tr->bucket = b;
-                    tr->tail_start = tail_start;
-                    tr->pad = 0;
-                }
-                if (lane < 32u)
-                    tr->hc[lane] = hc;
-            } else if (lane == 0) {
-                atomicOr(&st->err, 1u); // more allocations than the pool holds
-            }
-        }
-    }
-}
-
-__global__ void k_trig_keys(const qs_trig *__restrict__ trig, uint32_t n, uint32_t *__restrict__ tk,
-                            uint32_t *__restrict__ tv)
-{
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) {
-        tk[r] = trig[r].idx;
-        tv[r] = r;
-    }
-}
-
-// Insert step 4: ids in stream order of the triggering inserts (the sequential
-// bump allocator, v3:68-85, with next starting at 1025).
-__global__ __launch_bounds__(256) void k_assign(const qs_trig *__restrict__ trig,
-                                                const uint32_t *__restrict__ order, uint32_t n_trig,
-                                                uint32_t *__restrict__ low, uint32_t *__restrict__ high,
-                                                uint32_t *__restrict__ tail_of, uint64_t n_high,
-                                                qs_dev_state *__restrict__ st)
-{
-    const uint32_t base = st->next;
-    if ((uint64_t)(base - QS_LINK_START) + n_trig > n_high) {
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-            atomicOr(&st->err, 1u);
-        return;
-    }
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t gw = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const uint32_t nw = gridDim.x * 4u;
-    for (uint32_t r = gw; r < n_trig; r += nw) {
-        const qs_trig *tr = trig + order[r];
-        uint32_t id = base + r;
-        uint32_t row = id - QS_LINK_START;
-        if (lane < 32u)
-            high[(size_t)row * 32u + lane] = tr->hc[lane];
-        if (lane == 0) {
-            low[(size_t)tr->bucket * 8u + 7u] = id;
-            tail_of[row] = tr->tail_start;
-        }
-    }
-}
-
-// Insert step 5: fix-up inserts after the allocation's 64-insert batch are
-// plain high adds; a wavefront whose active lanes share one bucket (the usual
-// case: a long hot segment) adds popcount(B_j) once per counter.
-template <int KB>
-__global__ __launch_bounds__(256) void k_tail(const KeyT<KB> *__restrict__ keys,
-                                              const uint32_t *__restrict__ rb,
-                                              const uint32_t *__restrict__ ri, uint32_t r_n,
-                                              const uint32_t *__restrict__ low,
-                                              uint32_t *__restrict__ high,
-                                              const uint32_t *__restrict__ masks, qs_hashpar hp,
-                                              const uint32_t *__restrict__ tail_of)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < r_n; base += stride) {
-        uint32_t p = base + threadIdx.x;
-        bool act = false;
-        uint32_t b = 0, w = 0, M = 0;
-        if (p < r_n) {
-            b = rb[p];
-            w = low[(size_t)b * 8u + 7u];
-            if (w > QS_LINK_START && p >= tail_of[w - QS_LINK_START]) {
-                uint32_t bb, mu;
-                qs_hash_index(keys[ri[p]], hp, bb, mu);
-                M = masks[mu];
-                act = true;
-            }
-        }
-        uint64_t am = __ballot(act);
-        if (am == 0)
-            continue;
-        uint32_t first = (uint32_t)__builtin_ctzll(am);
-        uint32_t b0 = __shfl(b, (int)first, 64);
-        uint32_t w0 = __shfl(w, (int)first, 64);
-        uint64_t same = __ballot(act && b == b0);
-        if (same == am) {
-            uint64_t Bj = 0;
-#pragma unroll
-            for (int j = 0; j < 28; ++j) {
-                uint64_t bal = __ballot(act && ((M >> j) & 1u));
-                Bj = (lane == (uint32_t)j) ? bal : Bj;
-            }
-            uint32_t cnt = (uint32_t)__popcll(Bj);
-            if (lane < 28u && cnt)
-                atomicAdd(high + (size_t)(w0 - QS_LINK_START) * 32u + lane, cnt);
-        } else if (act) {
-            uint32_t *g = high + (size_t)(w - QS_LINK_START) * 32u;
-            uint32_t m = M;
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                atomicAdd(g + __builtin_ctz(m), 1u);
-                m &= m - 1u;
-            }
-        }
-    }
-}
-
-__global__ void k_finalize(qs_dev_state *st, uint32_t n_trig)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (st->err == 0)
-            st->next += n_trig;
-        st->any_viol = 0;
-        st->r_n = 0;
-        st->n_heads = 0;
-        st->n_trig = 0;
-    }
-}
-
-// ===========================================================================
 // Host: create / destroy / scratch
 // ===========================================================================
 
 static int g_cu_count = 0;
-
-static void free_scratch(qs_sketch *s)
-{
-    hipFree(s->d_tile_cnt);
-    hipFree(s->d_tile_off);
-    for (int k = 0; k < 2; ++k) {
-        hipFree(s->d_rb[k]);
-        hipFree(s->d_ri[k]);
-        hipFree(s->d_tk[k]);
-        hipFree(s->d_tv[k]);
-        s->d_rb[k] = s->d_ri[k] = s->d_tk[k] = s->d_tv[k] = nullptr;
-    }
-    hipFree(s->d_hist);
-    hipFree(s->d_heads);
-    hipFree(s->d_trig);
-    s->d_tile_cnt = s->d_tile_off = s->d_hist = s->d_heads = nullptr;
-    s->d_trig = nullptr;
-    s->chunk = 0;
-}
-
-static int ensure_scratch(qs_sketch *s, uint32_t chunk)
-{
-    if (s->chunk >= chunk)
-        return QS_OK;
-    free_scratch(s);
-    uint32_t tiles = (chunk + RB_TILE - 1) / RB_TILE;
-    uint64_t tcap = s->p.n_high < (uint64_t)chunk ? s->p.n_high : (uint64_t)chunk;
-    s->trig_cap = (uint32_t)tcap;
-    size_t hw = qs_sort_hist_words(chunk);
-    CK(hipMalloc(&s->d_tile_cnt, sizeof(uint32_t) * tiles));
-    CK(hipMalloc(&s->d_tile_off, sizeof(uint32_t) * tiles));
-    for (int k = 0; k < 2; ++k) {
-        CK(hipMalloc(&s->d_rb[k], sizeof(uint32_t) * chunk));
-        CK(hipMalloc(&s->d_ri[k], sizeof(uint32_t) * chunk));
-        CK(hipMalloc(&s->d_tk[k], sizeof(uint32_t) * s->trig_cap));
-        CK(hipMalloc(&s->d_tv[k], sizeof(uint32_t) * s->trig_cap));
-    }
-    CK(hipMalloc(&s->d_hist, sizeof(uint32_t) * hw));
-    CK(hipMalloc(&s->d_heads, sizeof(uint32_t) * chunk));
-    CK(hipMalloc(&s->d_trig, sizeof(qs_trig) * s->trig_cap));
-    s->chunk = chunk;
-    s->tile_n = tiles;
-    return QS_OK;
-}
 
 static int reset_state(qs_sketch *s, uint32_t next)
 {
@@ -789,12 +347,18 @@ extern "C" void qs_destroy(qs_sketch *s)
 {
     if (!s)
         return;
-    free_scratch(s);
+    if (s->tim) {
+        hipDeviceSynchronize();
+        tim_resolve(s);
+        delete (qs_timer *)s->tim;
+        s->tim = nullptr;
+    }
+    qs_free_scratch(s);
     hipFree(s->d_low);
     hipFree(s->d_high);
     hipFree(s->d_masks);
     hipFree(s->d_st);
-    hipFree(s->d_viol);
+    hipFree(s->d_longbm);
     hipFree(s->d_tail_of);
     if (s->h_st)
         hipHostFree(s->h_st);
@@ -859,11 +423,12 @@ extern "C" int qs_create(const qs_params *p, qs_sketch **out)
 
     size_t low_bytes = (size_t)p->n_low * 32u, high_bytes = (size_t)p->n_high * 128u;
     size_t viol_words = (size_t)((p->n_low + 31) / 32);
+    s->bucket_bits = p->n_low > 1 ? 64 - __builtin_clzll(p->n_low - 1) : 0;
     bool ok = hipMalloc(&s->d_low, low_bytes) == hipSuccess &&
               hipMalloc(&s->d_high, high_bytes) == hipSuccess &&
               hipMalloc(&s->d_masks, sizeof(uint32_t) * p->n_masks) == hipSuccess &&
               hipMalloc(&s->d_st, sizeof(qs_dev_state)) == hipSuccess &&
-              hipMalloc(&s->d_viol, sizeof(uint32_t) * viol_words) == hipSuccess &&
+              hipMalloc(&s->d_longbm, sizeof(uint32_t) * viol_words) == hipSuccess &&
               hipMalloc(&s->d_tail_of, sizeof(uint32_t) * p->n_high) == hipSuccess &&
               hipHostMalloc(&s->h_st, sizeof(qs_dev_state), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
@@ -872,7 +437,7 @@ extern "C" int qs_create(const qs_params *p, qs_sketch **out)
     }
     if (hipMemset(s->d_low, 0, low_bytes) != hipSuccess ||
         hipMemset(s->d_high, 0, high_bytes) != hipSuccess ||
-        hipMemset(s->d_viol, 0, sizeof(uint32_t) * viol_words) != hipSuccess ||
+        hipMemset(s->d_longbm, 0, sizeof(uint32_t) * viol_words) != hipSuccess ||
         hipMemcpy(s->d_masks, s->h_masks, sizeof(uint32_t) * p->n_masks, hipMemcpyHostToDevice) !=
             hipSuccess ||
         reset_state(s, QS_LINK_START + 1) != QS_OK || hipDeviceSynchronize() != hipSuccess) {
@@ -890,7 +455,7 @@ extern "C" int qs_clear(qs_sketch *s)
     CK(hipDeviceSynchronize());
     CK(hipMemset(s->d_low, 0, (size_t)s->p.n_low * 32u));
     CK(hipMemset(s->d_high, 0, (size_t)s->p.n_high * 128u));
-    CK(hipMemset(s->d_viol, 0, sizeof(uint32_t) * (size_t)((s->p.n_low + 31) / 32)));
+    CK(hipMemset(s->d_longbm, 0, sizeof(uint32_t) * (size_t)((s->p.n_low + 31) / 32)));
     int rc = reset_state(s, QS_LINK_START + 1);
     if (rc)
         return rc;
@@ -909,116 +474,6 @@ static uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t per_cu)
     return blocks ? (uint32_t)blocks : 1u;
 }
 
-static uint64_t default_chunk(const qs_sketch *s)
-{
-    return s->p.chunk_keys ? s->p.chunk_keys : (1ull << 24);
-}
-
-// Chunk schedule: a chunk is never larger than what the sketch has already
-// absorbed (floor 2^20), so the first chunks of a fresh sketch -- where hot
-// keys allocate and the fix-up is large -- stay small.  Any schedule gives the
-// same state; only the cost changes.
-static uint32_t next_chunk(const qs_sketch *s, size_t remaining)
-{
-    uint64_t cap = default_chunk(s);
-    uint64_t c = s->keys_inserted;
-    if (c < (1ull << 20))
-        c = 1ull << 20;
-    if (c > cap)
-        c = cap;
-    if (c > remaining)
-        c = remaining;
-    if (c > 0x7fffffffull)
-        c = 0x7fffffffull;
-    return (uint32_t)c;
-}
-
-template <int KB>
-static int fixup_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, hipStream_t stream)
-{
-    const uint32_t tiles = (cn + RB_TILE - 1) / RB_TILE;
-    hipLaunchKernelGGL(k_rollback<KB>, dim3(tiles), dim3(RB_THREADS), 0, stream, keys, cn, s->d_low,
-                       s->d_masks, s->hp, s->d_viol, s->d_tile_cnt);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, stream, s->d_tile_cnt, s->d_tile_off,
-                       tiles, &s->d_st->r_n);
-    hipLaunchKernelGGL(k_compact<KB>, dim3(tiles), dim3(RB_THREADS), 0, stream, keys, cn, s->d_low,
-                       s->hp, s->d_viol, s->d_tile_cnt, s->d_tile_off, s->d_rb[0], s->d_ri[0]);
-    CK(hipGetLastError());
-    CK(hipMemcpyAsync(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost, stream));
-    CK(hipStreamSynchronize(stream));
-    const uint32_t r_n = s->h_st->r_n;
-    s->fix_keys += r_n;
-    if (r_n == 0)
-        return QS_OK;
-    int bits = 32 - __builtin_clz((uint32_t)(s->p.n_low > 1 ? s->p.n_low - 1 : 1));
-    int cur = qs_sort_pairs(s->d_rb, s->d_ri, s->d_hist, r_n, bits, stream);
-    if (cur < 0)
-        return QS_ERR_HIP;
-    const uint32_t *rb = s->d_rb[cur], *ri = s->d_ri[cur];
-    hipLaunchKernelGGL(k_heads, dim3((r_n + 255) / 256), dim3(256), 0, stream, rb, r_n, s->d_heads,
-                       s->d_st);
-    hipLaunchKernelGGL(k_ordered<KB>, dim3(grid_for((uint64_t)r_n * 64, 256, 8)), dim3(256), 0,
-                       stream, keys, rb, ri, r_n, s->d_heads, s->d_low, s->d_masks, s->hp, s->d_viol,
-                       s->d_trig, s->trig_cap, s->d_st, s->p.semantics == QS_PAPER ? 1 : 0);
-    CK(hipGetLastError());
-    CK(hipMemcpyAsync(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost, stream));
-    CK(hipStreamSynchronize(stream));
-    uint32_t n_trig = s->h_st->n_trig;
-    if (n_trig > s->trig_cap)
-        n_trig = s->trig_cap;
-    if (n_trig) {
-        hipLaunchKernelGGL(k_trig_keys, dim3((n_trig + 255) / 256), dim3(256), 0, stream, s->d_trig,
-                           n_trig, s->d_tk[0], s->d_tv[0]);
-        int tbits = 32 - __builtin_clz(cn > 1 ? cn - 1 : 1);
-        int tc = qs_sort_pairs(s->d_tk, s->d_tv, s->d_hist, n_trig, tbits, stream);
-        if (tc < 0)
-            return QS_ERR_HIP;
-        hipLaunchKernelGGL(k_assign, dim3(grid_for((uint64_t)n_trig * 64, 256, 8)), dim3(256), 0,
-                           stream, s->d_trig, s->d_tv[tc], n_trig, s->d_low, s->d_high, s->d_tail_of,
-                           (uint64_t)s->p.n_high, s->d_st);
-        hipLaunchKernelGGL(k_tail<KB>, dim3(grid_for(r_n, 256, 8)), dim3(256), 0, stream, keys, rb, ri,
-                           r_n, s->d_low, s->d_high, s->d_masks, s->hp, s->d_tail_of);
-    }
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, stream, s->d_st, n_trig);
-    CK(hipGetLastError());
-    return QS_OK;
-}
-
-template <int KB>
-static int insert_impl(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream)
-{
-    const KeyT<KB> *keys = (const KeyT<KB> *)d_keys;
-    size_t done = 0;
-    while (done < n) {
-        uint32_t cn = next_chunk(s, n - done);
-        int rc = ensure_scratch(s, cn);
-        if (rc)
-            return rc;
-        const KeyT<KB> *ck = keys + done;
-        hipLaunchKernelGGL(k_insert<KB>, dim3(grid_for(cn, INS_THREADS, 8)), dim3(INS_THREADS), 0,
-                           stream, ck, cn, s->d_low, s->d_high, s->d_masks, s->hp, s->d_viol, s->d_st);
-        CK(hipGetLastError());
-        CK(hipMemcpyAsync(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost, stream));
-        CK(hipStreamSynchronize(stream));
-        if (s->h_st->any_viol) {
-            rc = fixup_chunk<KB>(s, ck, cn, stream);
-            if (rc)
-                return rc;
-            s->chunks_fix++;
-            CK(hipMemcpyAsync(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost, stream));
-            CK(hipStreamSynchronize(stream));
-        }
-        s->chunks++;
-        s->keys_inserted += cn;
-        done += cn;
-        if (s->h_st->err) {
-            s->sticky_err = QS_ERR_POOL;
-            return s->sticky_err;
-        }
-    }
-    return QS_OK;
-}
-
 static int insert_dispatch(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream)
 {
     if (!s || (!d_keys && n))
@@ -1027,8 +482,7 @@ static int insert_dispatch(qs_sketch *s, const void *d_keys, size_t n, hipStream
         return s->sticky_err;
     if (n == 0)
         return QS_OK;
-    return s->p.key_bytes == 4 ? insert_impl<4>(s, d_keys, n, stream)
-                               : insert_impl<8>(s, d_keys, n, stream);
+    return qs_insert_chunks(s, d_keys, n, stream);
 }
 
 extern "C" int qs_insert(qs_sketch *s, const void *d_keys, size_t n)
@@ -1055,6 +509,8 @@ static int query_dispatch(qs_sketch *s, const void *d_keys, size_t n, uint32_t *
     if (n > 0xffffffffull)
         return QS_ERR_PARAM;
     uint32_t nn = (uint32_t)n;
+    qs_evpair ev;
+    qs_tim_begin(s, ev, QS_TK_QUERY, nn, stream);
     if (s->p.key_bytes == 4)
         hipLaunchKernelGGL(k_query<4>, dim3(grid_for(nn, INS_THREADS, 8)), dim3(INS_THREADS), 0, stream,
                            (const uint32_t *)d_keys, nn, s->d_low, s->d_high, s->d_masks, s->hp,
@@ -1063,6 +519,7 @@ static int query_dispatch(qs_sketch *s, const void *d_keys, size_t n, uint32_t *
         hipLaunchKernelGGL(k_query<8>, dim3(grid_for(nn, INS_THREADS, 8)), dim3(INS_THREADS), 0, stream,
                            (const uint64_t *)d_keys, nn, s->d_low, s->d_high, s->d_masks, s->hp,
                            d_counts);
+    qs_tim_end(s, ev, stream);
     CK(hipGetLastError());
     return QS_OK;
 }
@@ -1138,13 +595,36 @@ extern "C" int qs_import_state(qs_sketch *s, const uint32_t *low, const uint32_t
     CK(hipMemcpy(s->d_low, low, (size_t)s->p.n_low * 32u, hipMemcpyHostToDevice));
     CK(hipMemcpy(s->d_high, high, (size_t)n_high * 128u, hipMemcpyHostToDevice));
     CK(hipMemcpy(s->d_masks, masks, sizeof(uint32_t) * s->p.n_masks, hipMemcpyHostToDevice));
-    CK(hipMemset(s->d_viol, 0, sizeof(uint32_t) * (size_t)((s->p.n_low + 31) / 32)));
+    CK(hipMemset(s->d_longbm, 0, sizeof(uint32_t) * (size_t)((s->p.n_low + 31) / 32)));
     int rc = reset_state(s, next);
     if (rc)
         return rc;
     CK(hipDeviceSynchronize());
     s->sticky_err = 0;
     s->keys_inserted = 0;
+    return QS_OK;
+}
+
+extern "C" int qs_set_timing(qs_sketch *s, int enable)
+{
+    if (!s)
+        return QS_ERR_PARAM;
+    CK(hipDeviceSynchronize());
+    if (!s->tim)
+        s->tim = new qs_timer();
+    tim_resolve(s);
+    memset(&s->tot, 0, sizeof(s->tot));
+    s->timing = enable ? 1 : 0;
+    return QS_OK;
+}
+
+extern "C" int qs_get_timing(qs_sketch *s, qs_timing *t)
+{
+    if (!s || !t)
+        return QS_ERR_PARAM;
+    CK(hipDeviceSynchronize());
+    tim_resolve(s);
+    *t = s->tot;
     return QS_OK;
 }
 

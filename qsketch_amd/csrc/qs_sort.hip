@@ -176,6 +176,12 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(
     }
 }
 
+void qs_scan_digits(uint32_t *hist, uint32_t ntiles, uint32_t nbins, uint32_t *totals,
+                    hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_sort_scan_digit, dim3(nbins), dim3(1024), 0, stream, hist, ntiles, totals);
+}
+
 int qs_sort_pairs(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist, uint32_t count,
                   int key_bits, hipStream_t stream)
 {
