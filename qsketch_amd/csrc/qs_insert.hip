@@ -14,12 +14,18 @@
 // size, and atomics on one 128-byte line serialise at ~11 ns.  An atomic per
 // key is therefore bounded far below the HBM roofline, and a Zipf-hot key turns
 // its three counters into a serial queue.  A stable LSD radix sort of the
-// chunk by bucket (the bucket is recomputed from the key in every pass, the
-// chunk-local stream index rides along) turns the table update into an
-// address-ordered sweep with one plain read-modify-write per touched bucket.
+// chunk by bucket turns the table update into an address-ordered sweep with one
+// plain read-modify-write per touched bucket.
+//
+// Each insert is hashed once, in the first radix pass, into a 64-bit item
+// (bucket | mask index | chunk-local stream index); later passes and kernels
+// only extract bit fields.  Digit ranks inside a wavefront come from an LDS
+// match (every lane ORs its lane bit into the slot of its digit and reads the
+// group mask back), which is stable and costs three LDS operations per item.
 //
 // Per chunk:
-//   sort      4-8 radix passes of (key, index) by bucket digit (8 bits/pass)
+//   sort      per pass: k_bs_up (tile digit counts), a scan over tiles,
+//             k_bs_down (rank, stage digit-sorted in LDS, write digit runs)
 //   k_seg     one wavefront per window of 64 sorted items: a segmented scan
 //             sums the packed byte increments of each bucket run; the run's
 //             last lane owns it.  Linked bucket: add to the high row.  Unlinked
@@ -31,9 +37,9 @@
 //   k_ordered one wavefront per queued segment replays it in stream order until
 //             its allocation (at most ceil(28*129/3) inserts can precede it).
 //   k_assign  allocation ids in stream order of the triggering inserts.
-//   k_tail    long segments after their allocation point: per-counter ballot
-//             popcounts over 16K-item windows, one atomic per counter per
-//             window and bucket.
+//   k_tail    long segments after their allocation point: per-lane LDS counter
+//             rows over 16K-item windows, one atomic per counter per window and
+//             bucket.
 #include "qs_internal.h"
 
 #include <cstring>
@@ -45,11 +51,6 @@
             return QS_ERR_HIP;                                                                   \
     } while (0)
 
-#define BS_THREADS 256
-#define BS_ITEMS 16
-#define BS_TILE (BS_THREADS * BS_ITEMS)
-#define BS_DBITS 8
-#define BS_BINS (1u << BS_DBITS)
 #define SEG_THREADS 256
 #define TAIL_WINDOW 16384u
 #define NO_BUCKET 0xffffffffu
@@ -76,126 +77,151 @@ static uint32_t grid_cap(uint64_t want, uint32_t per_cu)
 // ===========================================================================
 // radix sort of the chunk by bucket
 // ===========================================================================
-template <int KB>
-__global__ __launch_bounds__(BS_THREADS) void k_bsort_up(const KeyT<KB> *__restrict__ kin, uint32_t n,
-                                                         qs_hashpar hp, int shift, uint32_t ntiles,
-                                                         uint32_t *__restrict__ hist)
+#define OS_THREADS 256
+#define OS_WAVES 4
+#define OS_IPL 16                         // items per lane
+#define OS_WAVE_ITEMS (64 * OS_IPL)       // 1024 items per wave
+#define OS_TILE (OS_THREADS * OS_IPL)     // 4096 items per tile
+#define OS_BINS 256                       // LDS capacity for digits up to 8 bits
+
+template <int KB, bool FIRST>
+__device__ static inline uint64_t load_item(const void *__restrict__ in, uint32_t i,
+                                            const qs_hashpar &hp, const qs_pack &pk)
 {
-    __shared__ uint32_t s_h[BS_BINS];
-    for (uint32_t d = threadIdx.x; d < BS_BINS; d += BS_THREADS)
-        s_h[d] = 0;
-    __syncthreads();
-    const uint32_t base = blockIdx.x * BS_TILE;
-#pragma unroll 4
-    for (int r = 0; r < BS_ITEMS; ++r) {
-        uint32_t i = base + r * BS_THREADS + threadIdx.x;
-        if (i < n)
-            atomicAdd(&s_h[(qs_bucket(kin[i], hp) >> shift) & (BS_BINS - 1u)], 1u);
+    if (FIRST) {
+        const KeyT<KB> key = reinterpret_cast<const KeyT<KB> *>(in)[i];
+        return (uint64_t)qs_bucket(key, hp) | ((uint64_t)qs_maskidx(key, hp) << pk.bb) |
+               ((uint64_t)i << pk.ishift);
     }
-    __syncthreads();
-    for (uint32_t d = threadIdx.x; d < BS_BINS; d += BS_THREADS)
-        hist[(size_t)d * ntiles + blockIdx.x] = s_h[d];
+    return reinterpret_cast<const uint64_t *>(in)[i];
 }
 
-// Stable scatter of one tile: rank every item within the tile by digit
-// (per-wave digit match with BS_DBITS ballots, per-digit prefix over the
-// waves and the rounds), stage the digit-sorted tile in LDS, then write each
-// digit run contiguously to its global position.
-template <int KB>
-__global__ __launch_bounds__(BS_THREADS) void k_bsort_down(
-    const KeyT<KB> *__restrict__ kin, const uint32_t *__restrict__ iin, uint32_t n, qs_hashpar hp,
-    int shift, uint32_t ntiles, const uint32_t *__restrict__ hist,
-    const uint32_t *__restrict__ totals, KeyT<KB> *__restrict__ kout, uint32_t *__restrict__ iout)
+// Per-tile digit counts, digit-major: hist[d * ntiles + tile].
+template <int KB, bool FIRST>
+__global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ in, uint32_t n,
+                                                      qs_hashpar hp, qs_pack pk, int shift,
+                                                      uint32_t dmask, uint32_t nbins,
+                                                      uint32_t ntiles, uint32_t *__restrict__ hist)
 {
-    __shared__ KeyT<KB> s_key[BS_TILE];
-    __shared__ uint32_t s_idx[BS_TILE];
-    __shared__ uint32_t s_off[BS_BINS];   // global position of local slot 0 of digit d
-    __shared__ uint32_t s_run[BS_BINS];   // next local slot of digit d
-    __shared__ uint32_t s_wc[4][BS_BINS]; // per-wave digit counts of a round
-    __shared__ uint32_t s_wo[4][BS_BINS]; // per-wave digit offsets of a round
-    __shared__ uint32_t s_red[2][4];
+    __shared__ uint32_t s_h[OS_BINS];
+    const uint32_t t = threadIdx.x;
+    s_h[t] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * OS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < OS_IPL; ++r) {
+        const uint32_t i = base + r * OS_THREADS + t;
+        if (i < n) {
+            uint32_t b;
+            if (FIRST)
+                b = qs_bucket(reinterpret_cast<const KeyT<KB> *>(in)[i], hp);
+            else
+                b = it_b(reinterpret_cast<const uint64_t *>(in)[i], pk);
+            atomicAdd(&s_h[(b >> shift) & dmask], 1u);
+        }
+    }
+    __syncthreads();
+    if (t < nbins)
+        hist[(size_t)t * ntiles + blockIdx.x] = s_h[t];
+}
+
+// exclusive block scan of one value per thread (256 threads)
+__device__ static inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_red)
+{
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o)
+            x += y;
+    }
+    if (lane == 63)
+        s_red[wv] = x;
+    __syncthreads();
+    uint32_t off = 0;
+#pragma unroll
+    for (int k = 0; k < OS_WAVES; ++k)
+        off += (k < (int)wv) ? s_red[k] : 0u;
+    __syncthreads();
+    return off + x - v;
+}
+
+// Stable scatter of one tile.  Wave w ranks its 1024 consecutive items in 16
+// rounds: the LDS match gives the lanes sharing a digit, the lowest of them
+// advances the wave's running count.  The tile is then staged digit-sorted in
+// LDS and written out as contiguous digit runs.
+template <int KB, bool FIRST>
+__global__ __launch_bounds__(OS_THREADS) void k_bs_down(
+    const void *__restrict__ in, uint32_t n, qs_hashpar hp, qs_pack pk, int shift, uint32_t dmask,
+    uint32_t nbins, uint32_t ntiles, const uint32_t *__restrict__ hist,
+    const uint32_t *__restrict__ totals, uint64_t *__restrict__ out)
+{
+    __shared__ uint64_t s_item[OS_TILE];
+    __shared__ unsigned long long s_match[OS_WAVES][OS_BINS];
+    __shared__ uint32_t s_wh[OS_WAVES][OS_BINS];
+    __shared__ uint32_t s_ls[OS_BINS];
+    __shared__ uint32_t s_goff[OS_BINS];
+    __shared__ uint32_t s_red[OS_WAVES];
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
     const uint32_t tile = blockIdx.x;
-    const uint32_t base = tile * BS_TILE;
-
-    // digit d = t (BS_BINS == BS_THREADS): global base and local start
-    {
-        uint32_t tot = totals[t];
-        uint32_t hcur = hist[(size_t)t * ntiles + tile];
-        uint32_t hnxt = (tile + 1 < ntiles) ? hist[(size_t)t * ntiles + tile + 1] : tot;
-        uint32_t cnt = hnxt - hcur;
-        uint32_t xa = tot, xb = cnt;
+    const uint32_t base = tile * OS_TILE;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t ya = __shfl_up(xa, o, 64), yb = __shfl_up(xb, o, 64);
-            if (lane >= (uint32_t)o) {
-                xa += ya;
-                xb += yb;
-            }
-        }
-        if (lane == 63) {
-            s_red[0][wv] = xa;
-            s_red[1][wv] = xb;
-        }
-        __syncthreads();
-        uint32_t oa = 0, ob = 0;
-        for (uint32_t k = 0; k < wv; ++k) {
-            oa += s_red[0][k];
-            ob += s_red[1][k];
-        }
-        uint32_t dexcl = oa + xa - tot;  // digits before t, all tiles
-        uint32_t lstart = ob + xb - cnt; // digits before t, this tile
-        s_off[t] = dexcl + hcur - lstart;
-        s_run[t] = lstart;
-#pragma unroll
-        for (int w = 0; w < 4; ++w)
-            s_wc[w][t] = 0;
+    for (int w = 0; w < OS_WAVES; ++w) {
+        s_wh[w][t] = 0;
+        s_match[w][t] = 0;
     }
+    const bool dv = t < nbins;
+    const uint32_t tot = dv ? totals[t] : 0u;
+    const uint32_t hcur = dv ? hist[(size_t)t * ntiles + tile] : 0u;
+    const uint32_t hnxt = dv ? ((tile + 1 < ntiles) ? hist[(size_t)t * ntiles + tile + 1] : tot) : 0u;
+    const uint32_t dbase = block_excl_scan256(tot, s_red);
+    const uint32_t lstart = block_excl_scan256(hnxt - hcur, s_red);
+    s_ls[t] = lstart;
+    s_goff[t] = dbase + hcur - lstart;
     __syncthreads();
 
     const uint64_t lt = (1ull << lane) - 1ull;
-    for (int r = 0; r < BS_ITEMS; ++r) {
-        uint32_t i = base + r * BS_THREADS + t;
-        bool valid = i < n;
-        KeyT<KB> key = valid ? kin[i] : (KeyT<KB>)0;
-        uint32_t idx = valid ? (iin ? iin[i] : i) : 0u;
-        uint32_t d = (qs_bucket(key, hp) >> shift) & (BS_BINS - 1u);
-        uint64_t m = __ballot(valid);
+    uint64_t item[OS_IPL];
+    uint32_t dr[OS_IPL]; // digit | wave-local position << 8 | valid << 31
 #pragma unroll
-        for (int bit = 0; bit < BS_DBITS; ++bit) {
-            uint64_t bb = __ballot((d >> bit) & 1u);
-            m &= ((d >> bit) & 1u) ? bb : ~bb;
+    for (int r = 0; r < OS_IPL; ++r) {
+        const uint32_t i = base + wv * OS_WAVE_ITEMS + r * 64u + lane;
+        const bool valid = i < n;
+        item[r] = valid ? load_item<KB, FIRST>(in, i, hp, pk) : 0ull;
+        const uint32_t d = (it_b(item[r], pk) >> shift) & dmask;
+        if (valid)
+            atomicOr(&s_match[wv][d], (unsigned long long)(1ull << lane));
+        const uint64_t m = s_match[wv][d];
+        const uint32_t rank = (uint32_t)__popcll(m & lt);
+        const uint32_t run = s_wh[wv][d];
+        if (valid && rank == 0) {
+            s_wh[wv][d] = run + (uint32_t)__popcll(m);
+            s_match[wv][d] = 0;
         }
-        uint32_t rank = (uint32_t)__popcll(m & lt);
-        if (valid && rank == 0)
-            s_wc[wv][d] = (uint32_t)__popcll(m);
-        __syncthreads();
-        {
-            uint32_t run = s_run[t];
+        dr[r] = d | ((run + rank) << 8) | (valid ? 0x80000000u : 0u);
+    }
+    __syncthreads();
+    {
+        uint32_t c0 = s_wh[0][t], c1 = s_wh[1][t], c2 = s_wh[2][t];
+        s_wh[0][t] = 0;
+        s_wh[1][t] = c0;
+        s_wh[2][t] = c0 + c1;
+        s_wh[3][t] = c0 + c1 + c2;
+    }
+    __syncthreads();
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                uint32_t c = s_wc[w][t];
-                s_wo[w][t] = run;
-                s_wc[w][t] = 0;
-                run += c;
-            }
-            s_run[t] = run;
-        }
-        __syncthreads();
-        if (valid) {
-            uint32_t lpos = s_wo[wv][d] + rank;
-            s_key[lpos] = key;
-            s_idx[lpos] = idx;
+    for (int r = 0; r < OS_IPL; ++r) {
+        if (dr[r] & 0x80000000u) {
+            const uint32_t d = dr[r] & 0xffu;
+            s_item[s_ls[d] + s_wh[wv][d] + ((dr[r] >> 8) & 0x7fffffu)] = item[r];
         }
     }
     __syncthreads();
-    const uint32_t tn = (n - base) < BS_TILE ? (n - base) : BS_TILE;
-    for (uint32_t q = t; q < tn; q += BS_THREADS) {
-        KeyT<KB> key = s_key[q];
-        uint32_t d = (qs_bucket(key, hp) >> shift) & (BS_BINS - 1u);
-        uint32_t g = s_off[d] + q;
-        kout[g] = key;
-        iout[g] = s_idx[q];
+    const uint32_t tn = (n - base) < OS_TILE ? (n - base) : OS_TILE;
+    for (uint32_t q = t; q < tn; q += OS_THREADS) {
+        const uint64_t it = s_item[q];
+        out[s_goff[(it_b(it, pk) >> shift) & dmask] + q] = it;
     }
 }
 
@@ -220,15 +246,6 @@ __device__ static inline void mask_to_words(uint32_t M, uint32_t a[7])
                 a[k] += ((j >> 2) == (uint32_t)k) ? inc : 0u;
         }
     }
-}
-
-__device__ static inline uint32_t sel7(const uint32_t a[7], uint32_t k)
-{
-    uint32_t r = a[0];
-#pragma unroll
-    for (int q = 1; q < 7; ++q)
-        r = (k == (uint32_t)q) ? a[q] : r;
-    return r;
 }
 
 // Owner of a complete segment (packed per-byte counts a[], each <= 128):
@@ -295,15 +312,14 @@ __device__ static inline void wave_append(bool flag, uint32_t value, uint32_t *l
         list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
 }
 
-template <int KB>
 __global__ __launch_bounds__(SEG_THREADS) void k_seg(
-    const KeyT<KB> *__restrict__ K, uint32_t n, uint32_t *__restrict__ low,
-    uint32_t *__restrict__ high, const uint32_t *__restrict__ masks, qs_hashpar hp,
+    const uint64_t *__restrict__ IT, uint32_t n, uint32_t *__restrict__ low,
+    uint32_t *__restrict__ high, const uint32_t *__restrict__ masks, qs_pack pk, uint32_t n_masks,
     uint32_t *__restrict__ longbm, uint32_t *__restrict__ ord, uint32_t *__restrict__ lng,
     qs_dev_state *__restrict__ st)
 {
     __shared__ uint32_t s_masks[QS_MAX_MASKS];
-    for (uint32_t i = threadIdx.x; i < hp.n_masks; i += SEG_THREADS)
+    for (uint32_t i = threadIdx.x; i < n_masks; i += SEG_THREADS)
         s_masks[i] = masks[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
@@ -313,15 +329,18 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
     for (uint64_t w0 = (uint64_t)gw * 64u; w0 < n; w0 += (uint64_t)nw * 64u) {
         const uint32_t p = (uint32_t)w0 + lane;
         const bool valid = p < n;
-        KeyT<KB> key = valid ? K[p] : (KeyT<KB>)0;
-        uint32_t b = valid ? qs_bucket(key, hp) : NO_BUCKET;
-        uint32_t M = valid ? s_masks[qs_maskidx(key, hp)] : 0u;
+        const uint64_t it = valid ? IT[p] : 0ull;
+        const uint32_t b = valid ? it_b(it, pk) : NO_BUCKET;
         uint32_t bl = __shfl(b, (int)(lane == 0 ? 0 : lane - 1u), 64);
-        uint32_t br = __shfl(b, (int)(lane == 63 ? 63 : lane + 1u), 64);
         if (lane == 0)
-            bl = (w0 > 0) ? qs_bucket(K[w0 - 1], hp) : NO_BUCKET;
+            bl = (w0 > 0) ? it_b(IT[w0 - 1], pk) : NO_BUCKET;
+        // a window inside a segment that started earlier has nothing to own
+        if (__ballot(valid && b != bl) == 0)
+            continue;
+        const uint32_t M = valid ? s_masks[it_mu(it, pk)] : 0u;
+        uint32_t br = __shfl(b, (int)(lane == 63 ? 63 : lane + 1u), 64);
         if (lane == 63)
-            br = (p + 1u < n) ? qs_bucket(K[p + 1u], hp) : END_BUCKET;
+            br = (p + 1u < n) ? it_b(IT[p + 1u], pk) : END_BUCKET;
         const bool head = valid && b != bl;
         const bool last = valid && b != br;
         uint32_t a[7];
@@ -344,7 +363,6 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
         const uint64_t heads = __ballot(head);
         const uint32_t s = 63u - (uint32_t)__builtin_clzll(starts & le); // first lane of my run
         const bool owned = valid && (s > 0u || (heads & 1ull));
-        // complete runs owned by this window
         bool risky = false;
         if (owned && last)
             risky = apply_segment(low, high, b, a);
@@ -362,13 +380,12 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
             bool in = false;
             uint32_t Mq = 0;
             if (q < n) {
-                KeyT<KB> kq = K[q];
-                in = qs_bucket(kq, hp) == b63;
-                Mq = in ? s_masks[qs_maskidx(kq, hp)] : 0u;
+                const uint64_t iq = IT[q];
+                in = it_b(iq, pk) == b63;
+                Mq = in ? s_masks[it_mu(iq, pk)] : 0u;
             }
             const uint64_t im = __ballot(in);
             const uint32_t head_pos = (uint32_t)w0 + s63;
-            // the run is a prefix of the next window; its end is inside iff not full
             if (im != ~0ull) {
                 uint32_t c[7];
                 mask_to_words(Mq, c);
@@ -419,12 +436,10 @@ __device__ static inline uint32_t kth_set_bit(uint64_t m, uint32_t k)
 // Lane j (< 28) holds byte counter j; a batch of 64 inserts is transposed into
 // 28 ballots B_j, and the first insert that would see a selected byte > 128 is
 // the (129 - c_j)-th set bit of B_j, minimised over j.
-template <int KB>
-__global__ __launch_bounds__(256) void k_ordered(const KeyT<KB> *__restrict__ K,
-                                                 const uint32_t *__restrict__ I, uint32_t n,
+__global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT, uint32_t n,
                                                  const uint32_t *__restrict__ ord,
                                                  uint32_t *__restrict__ low,
-                                                 const uint32_t *__restrict__ masks, qs_hashpar hp,
+                                                 const uint32_t *__restrict__ masks, qs_pack pk,
                                                  uint32_t *__restrict__ longbm,
                                                  uint32_t *__restrict__ lng,
                                                  qs_trig *__restrict__ trig, uint32_t trig_cap,
@@ -436,7 +451,7 @@ __global__ __launch_bounds__(256) void k_ordered(const KeyT<KB> *__restrict__ K,
     const uint32_t n_ord = st->n_ord;
     for (uint32_t h = gw; h < n_ord; h += nw) {
         const uint32_t p0 = ord[h];
-        const uint32_t b = qs_bucket(K[p0], hp);
+        const uint32_t b = it_b(IT[p0], pk);
         uint32_t *bk = low + (size_t)b * 8u;
         uint32_t word = lane < 7u ? bk[lane] : 0u;
         uint32_t c = __shfl(word, (int)(lane >> 2), 64);
@@ -449,11 +464,11 @@ __global__ __launch_bounds__(256) void k_ordered(const KeyT<KB> *__restrict__ K,
             bool valid = false;
             uint32_t M = 0, idx = 0;
             if (q < n) {
-                KeyT<KB> kq = K[q];
-                valid = qs_bucket(kq, hp) == b;
+                const uint64_t iq = IT[q];
+                valid = it_b(iq, pk) == b;
                 if (valid) {
-                    M = masks[qs_maskidx(kq, hp)];
-                    idx = I[q];
+                    M = masks[it_mu(iq, pk)];
+                    idx = it_idx(iq, pk);
                 }
             }
             const uint64_t vm = __ballot(valid);
@@ -572,72 +587,127 @@ __global__ __launch_bounds__(256) void k_assign(const qs_trig *__restrict__ trig
 // Long segments: every insert of a bucket linked before this chunk, and the
 // inserts after the allocation batch of a bucket linked in this chunk, are
 // commutative high adds.  A wavefront walks a window of TAIL_WINDOW sorted
-// positions and keeps per-counter sums (lane j = counter j) while the bucket
-// stays the same, flushing them with one atomic per counter.
-template <int KB>
-__global__ __launch_bounds__(256) void k_tail(const KeyT<KB> *__restrict__ K, uint32_t n,
+// positions; each lane counts its inserts' counters in its own LDS row while
+// the link stays the same, and the rows are summed and flushed with one atomic
+// per counter when it changes.
+#define TAIL_ROW 29 // 28 counters, odd stride against bank conflicts
+
+__device__ static inline void tail_flush(uint32_t (*rows)[TAIL_ROW], uint32_t lane,
+                                         uint32_t *__restrict__ high, uint32_t cur_w)
+{
+    if (lane < 28u) {
+        uint32_t sum = 0;
+        for (uint32_t l = 0; l < 64u; ++l)
+            sum += rows[l][lane];
+        if (sum)
+            atomicAdd(high + (size_t)(cur_w - QS_LINK_START) * 32u + lane, sum);
+    }
+#pragma unroll
+    for (int j = 0; j < 28; ++j)
+        rows[lane][j] = 0;
+}
+
+__device__ static inline void tail_add(uint32_t (*rows)[TAIL_ROW], uint32_t lane, uint32_t M)
+{
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        if (M) {
+            rows[lane][__builtin_ctz(M)] += 1u;
+            M &= M - 1u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, uint32_t n,
                                               const uint32_t *__restrict__ low,
                                               uint32_t *__restrict__ high,
-                                              const uint32_t *__restrict__ masks, qs_hashpar hp,
-                                              const uint32_t *__restrict__ longbm,
+                                              const uint32_t *__restrict__ masks, qs_pack pk,
+                                              uint32_t n_masks, const uint32_t *__restrict__ longbm,
                                               const uint32_t *__restrict__ tail_of,
                                               const qs_dev_state *__restrict__ st)
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t gw = blockIdx.x * 4u + (threadIdx.x >> 6);
+    __shared__ uint32_t s_masks[QS_MAX_MASKS];
+    __shared__ uint32_t s_rows[4][64][TAIL_ROW];
+    for (uint32_t i = threadIdx.x; i < n_masks; i += blockDim.x)
+        s_masks[i] = masks[i];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t(*rows)[TAIL_ROW] = s_rows[wv];
+#pragma unroll
+    for (int j = 0; j < 28; ++j)
+        rows[lane][j] = 0;
+    __syncthreads();
+    const uint32_t gw = blockIdx.x * 4u + wv;
     const uint32_t nw = gridDim.x * 4u;
     const uint32_t first_new = st->next;
     for (uint64_t w0 = (uint64_t)gw * TAIL_WINDOW; w0 < n; w0 += (uint64_t)nw * TAIL_WINDOW) {
         const uint64_t wend = (w0 + TAIL_WINDOW < n) ? w0 + TAIL_WINDOW : n;
-        uint32_t cur_w = 0, acc = 0;
+        uint32_t cur_w = 0;
+        // lookup cache for the bucket of the previous single-bucket step
+        uint32_t c_b = NO_BUCKET, c_w = 0, c_tail = 0;
+        bool c_long = false;
         for (uint64_t s0 = w0; s0 < wend; s0 += 64u) {
             const uint32_t p = (uint32_t)s0 + lane;
-            bool act = false;
-            uint32_t b = NO_BUCKET, w = 0, M = 0;
-            if (p < wend) {
-                KeyT<KB> kp = K[p];
-                b = qs_bucket(kp, hp);
-                if ((longbm[b >> 5] >> (b & 31u)) & 1u) {
-                    w = low[(size_t)b * 8u + 7u];
-                    act = w > QS_LINK_START && (w < first_new || p >= tail_of[w - QS_LINK_START]);
-                    if (act)
-                        M = masks[qs_maskidx(kp, hp)];
+            const bool valid = p < wend;
+            const uint64_t it = valid ? IT[p] : 0ull;
+            const uint32_t b = valid ? it_b(it, pk) : NO_BUCKET;
+            const uint32_t b_first = __shfl(b, 0, 64);
+            const uint32_t last_lane = (wend - s0 < 64u) ? (uint32_t)(wend - s0 - 1u) : 63u;
+            const uint32_t b_last = __shfl(b, (int)last_lane, 64);
+            if (b_first == b_last) {
+                // the whole step is one bucket (the items are sorted by bucket)
+                if (b_first != c_b) {
+                    c_b = b_first;
+                    c_long = (longbm[c_b >> 5] >> (c_b & 31u)) & 1u;
+                    c_w = c_long ? low[(size_t)c_b * 8u + 7u] : 0u;
+                    c_tail = (c_w > QS_LINK_START && c_w >= first_new) ? tail_of[c_w - QS_LINK_START] : 0u;
                 }
+                if (!c_long || c_w <= QS_LINK_START)
+                    continue;
+                const bool act = valid && p >= c_tail;
+                if (!__ballot(act))
+                    continue;
+                if (c_w != cur_w) {
+                    if (cur_w)
+                        tail_flush(rows, lane, high, cur_w);
+                    cur_w = c_w;
+                }
+                if (act)
+                    tail_add(rows, lane, s_masks[it_mu(it, pk)]);
+                continue;
+            }
+            bool act = false;
+            uint32_t w = 0;
+            if (valid && ((longbm[b >> 5] >> (b & 31u)) & 1u)) {
+                w = low[(size_t)b * 8u + 7u];
+                act = w > QS_LINK_START && (w < first_new || p >= tail_of[w - QS_LINK_START]);
             }
             uint64_t am = __ballot(act);
             while (am) {
                 const uint32_t lead = (uint32_t)__builtin_ctzll(am);
                 const uint32_t wl = __shfl(w, (int)lead, 64);
                 const bool in = act && w == wl;
-                const uint64_t grp = __ballot(in);
                 if (wl != cur_w) {
-                    if (cur_w && lane < 28u && acc)
-                        atomicAdd(high + (size_t)(cur_w - QS_LINK_START) * 32u + lane, acc);
+                    if (cur_w)
+                        tail_flush(rows, lane, high, cur_w);
                     cur_w = wl;
-                    acc = 0;
                 }
-#pragma unroll
-                for (int j = 0; j < 28; ++j) {
-                    uint64_t bal = __ballot(in && ((M >> j) & 1u));
-                    if (lane == (uint32_t)j)
-                        acc += (uint32_t)__popcll(bal);
-                }
-                am &= ~grp;
+                if (in)
+                    tail_add(rows, lane, s_masks[it_mu(it, pk)]);
+                am &= ~__ballot(in);
             }
         }
-        if (cur_w && lane < 28u && acc)
-            atomicAdd(high + (size_t)(cur_w - QS_LINK_START) * 32u + lane, acc);
+        if (cur_w)
+            tail_flush(rows, lane, high, cur_w);
     }
 }
 
-__global__ void k_clear_long(const uint32_t *__restrict__ lng, const uint32_t *__restrict__ K32,
-                             const uint64_t *__restrict__ K64, qs_hashpar hp,
-                             uint32_t *__restrict__ longbm, const qs_dev_state *__restrict__ st)
+__global__ void k_clear_long(const uint32_t *__restrict__ lng, const uint64_t *__restrict__ IT,
+                             qs_pack pk, uint32_t *__restrict__ longbm,
+                             const qs_dev_state *__restrict__ st)
 {
     const uint32_t n_long = st->n_long;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_long; i += gridDim.x * blockDim.x) {
-        uint32_t p = lng[i];
-        uint32_t b = K32 ? qs_bucket(K32[p], hp) : qs_bucket(K64[p], hp);
+        const uint32_t b = it_b(IT[lng[i]], pk);
         atomicAnd(longbm + (b >> 5), ~(1u << (b & 31u)));
     }
 }
@@ -658,10 +728,10 @@ __global__ void k_finalize(qs_dev_state *st, uint32_t n_trig)
 // ===========================================================================
 uint64_t qs_scratch_hist_words(uint64_t chunk)
 {
-    uint64_t tiles = (chunk + BS_TILE - 1) / BS_TILE;
+    uint64_t tiles = (chunk + OS_TILE - 1) / OS_TILE;
     if (tiles == 0)
         tiles = 1;
-    uint64_t a = BS_BINS * tiles + BS_BINS;
+    uint64_t a = OS_BINS * tiles + OS_BINS;
     uint64_t b = qs_sort_hist_words((uint32_t)(chunk < 0xffffffffull ? chunk : 0xffffffffull));
     return a > b ? a : b;
 }
@@ -669,12 +739,11 @@ uint64_t qs_scratch_hist_words(uint64_t chunk)
 void qs_free_scratch(qs_sketch *s)
 {
     for (int k = 0; k < 2; ++k) {
-        hipFree(s->d_k[k]);
-        hipFree(s->d_i[k]);
+        hipFree(s->d_it[k]);
         hipFree(s->d_tk[k]);
         hipFree(s->d_tv[k]);
-        s->d_k[k] = nullptr;
-        s->d_i[k] = s->d_tk[k] = s->d_tv[k] = nullptr;
+        s->d_it[k] = nullptr;
+        s->d_tk[k] = s->d_tv[k] = nullptr;
     }
     hipFree(s->d_hist);
     hipFree(s->d_ord);
@@ -690,12 +759,10 @@ int qs_ensure_scratch(qs_sketch *s, uint64_t chunk)
     if (s->chunk >= chunk)
         return QS_OK;
     qs_free_scratch(s);
-    const size_t kb = s->p.key_bytes;
     uint64_t tcap = s->p.n_high < chunk ? s->p.n_high : chunk;
     s->trig_cap = (uint32_t)tcap;
     for (int k = 0; k < 2; ++k) {
-        if (hipMalloc(&s->d_k[k], kb * chunk) != hipSuccess ||
-            hipMalloc(&s->d_i[k], sizeof(uint32_t) * chunk) != hipSuccess ||
+        if (hipMalloc(&s->d_it[k], sizeof(uint64_t) * chunk) != hipSuccess ||
             hipMalloc(&s->d_tk[k], sizeof(uint32_t) * tcap) != hipSuccess ||
             hipMalloc(&s->d_tv[k], sizeof(uint32_t) * tcap) != hipSuccess)
             return QS_ERR_NOMEM;
@@ -712,31 +779,39 @@ int qs_ensure_scratch(qs_sketch *s, uint64_t chunk)
 template <int KB>
 static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, hipStream_t stream, int &res)
 {
-    const uint32_t ntiles = (cn + BS_TILE - 1) / BS_TILE;
-    uint32_t *totals = s->d_hist + (size_t)BS_BINS * ntiles;
-    int passes = (s->bucket_bits + BS_DBITS - 1) / BS_DBITS;
-    if (passes < 1)
-        passes = 1; // n_low == 1: one pass still yields the (identity) index array
-    KeyT<KB> *kb[2] = {(KeyT<KB> *)s->d_k[0], (KeyT<KB> *)s->d_k[1]};
-    const KeyT<KB> *kin = keys;
-    const uint32_t *iin = nullptr;
+    const uint32_t ntiles = (cn + OS_TILE - 1) / OS_TILE;
+    const int bits = (int)s->pk.bb;
+    const int passes = (bits + 7) / 8;
+    const int dbits = (bits + passes - 1) / passes; // <= 8
+    const uint32_t nbins = 1u << dbits, dmask = nbins - 1u;
+    uint32_t *totals = s->d_hist + (size_t)OS_BINS * ntiles;
+    const void *in = keys;
     int out = 0;
     for (int ps = 0; ps < passes; ++ps) {
-        const int shift = BS_DBITS * ps;
+        const int shift = dbits * ps;
         qs_evpair e0, e1, e2;
         qs_tim_begin(s, e0, QS_TK_SORT_UP, cn, stream);
-        hipLaunchKernelGGL(k_bsort_up<KB>, dim3(ntiles), dim3(BS_THREADS), 0, stream, kin, cn, s->hp,
-                           shift, ntiles, s->d_hist);
+        if (ps == 0)
+            hipLaunchKernelGGL((k_bs_up<KB, true>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+                               s->hp, s->pk, shift, dmask, nbins, ntiles, s->d_hist);
+        else
+            hipLaunchKernelGGL((k_bs_up<KB, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+                               s->hp, s->pk, shift, dmask, nbins, ntiles, s->d_hist);
         qs_tim_end(s, e0, stream);
         qs_tim_begin(s, e1, QS_TK_SORT_SCAN, cn, stream);
-        qs_scan_digits(s->d_hist, ntiles, BS_BINS, totals, stream);
+        qs_scan_digits(s->d_hist, ntiles, nbins, totals, stream);
         qs_tim_end(s, e1, stream);
         qs_tim_begin(s, e2, QS_TK_SORT_DOWN, cn, stream);
-        hipLaunchKernelGGL(k_bsort_down<KB>, dim3(ntiles), dim3(BS_THREADS), 0, stream, kin, iin, cn,
-                           s->hp, shift, ntiles, s->d_hist, totals, kb[out], s->d_i[out]);
+        if (ps == 0)
+            hipLaunchKernelGGL((k_bs_down<KB, true>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+                               s->hp, s->pk, shift, dmask, nbins, ntiles, s->d_hist, totals,
+                               s->d_it[out]);
+        else
+            hipLaunchKernelGGL((k_bs_down<KB, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+                               s->hp, s->pk, shift, dmask, nbins, ntiles, s->d_hist, totals,
+                               s->d_it[out]);
         qs_tim_end(s, e2, stream);
-        kin = kb[out];
-        iin = s->d_i[out];
+        in = s->d_it[out];
         out ^= 1;
     }
     CK(hipGetLastError());
@@ -754,32 +829,30 @@ static int read_state(qs_sketch *s, hipStream_t stream)
 template <int KB>
 static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, hipStream_t stream)
 {
-    int res = 0, rc;
-    rc = sort_chunk<KB>(s, keys, cn, stream, res);
+    int res = 0;
+    int rc = sort_chunk<KB>(s, keys, cn, stream, res);
     if (rc)
         return rc;
-    const KeyT<KB> *K = (const KeyT<KB> *)s->d_k[res];
-    const uint32_t *I = s->d_i[res];
+    const uint64_t *IT = s->d_it[res];
     qs_evpair es;
     qs_tim_begin(s, es, QS_TK_SEG, cn, stream);
-    hipLaunchKernelGGL(k_seg<KB>, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(SEG_THREADS), 0,
-                       stream, K, cn, s->d_low, s->d_high, s->d_masks, s->hp, s->d_longbm, s->d_ord,
-                       s->d_long, s->d_st);
+    hipLaunchKernelGGL(k_seg, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(SEG_THREADS), 0, stream,
+                       IT, cn, s->d_low, s->d_high, s->d_masks, s->pk, s->p.n_masks, s->d_longbm,
+                       s->d_ord, s->d_long, s->d_st);
     qs_tim_end(s, es, stream);
     CK(hipGetLastError());
     rc = read_state(s, stream);
     if (rc)
         return rc;
-    uint32_t n_ord = s->h_st->n_ord;
+    const uint32_t n_ord = s->h_st->n_ord;
     uint32_t n_trig = 0;
     if (n_ord) {
         s->chunks_fix++;
         qs_evpair eo;
         qs_tim_begin(s, eo, QS_TK_ORDERED, cn, stream);
-        hipLaunchKernelGGL(k_ordered<KB>, dim3(grid_cap(((uint64_t)n_ord + 3) / 4, 8)), dim3(256), 0,
-                           stream, K, I, cn, s->d_ord, s->d_low, s->d_masks, s->hp, s->d_longbm,
-                           s->d_long, s->d_trig, s->trig_cap, s->d_st,
-                           s->p.semantics == QS_PAPER ? 1 : 0);
+        hipLaunchKernelGGL(k_ordered, dim3(grid_cap(((uint64_t)n_ord + 3) / 4, 8)), dim3(256), 0, stream,
+                           IT, cn, s->d_ord, s->d_low, s->d_masks, s->pk, s->d_longbm, s->d_long,
+                           s->d_trig, s->trig_cap, s->d_st, s->p.semantics == QS_PAPER ? 1 : 0);
         qs_tim_end(s, eo, stream);
         CK(hipGetLastError());
         rc = read_state(s, stream);
@@ -806,12 +879,11 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, hipStre
     if (s->h_st->n_long) {
         qs_evpair et;
         qs_tim_begin(s, et, QS_TK_TAIL, cn, stream);
-        hipLaunchKernelGGL(k_tail<KB>, dim3(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8)),
-                           dim3(256), 0, stream, K, cn, s->d_low, s->d_high, s->d_masks, s->hp,
-                           s->d_longbm, s->d_tail_of, s->d_st);
+        hipLaunchKernelGGL(k_tail, dim3(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8)),
+                           dim3(256), 0, stream, IT, cn, s->d_low, s->d_high, s->d_masks, s->pk,
+                           s->p.n_masks, s->d_longbm, s->d_tail_of, s->d_st);
         hipLaunchKernelGGL(k_clear_long, dim3(grid_cap((s->h_st->n_long + 255) / 256, 4)), dim3(256), 0,
-                           stream, s->d_long, KB == 4 ? (const uint32_t *)K : nullptr,
-                           KB == 8 ? (const uint64_t *)K : nullptr, s->hp, s->d_longbm, s->d_st);
+                           stream, s->d_long, IT, s->pk, s->d_longbm, s->d_st);
         qs_tim_end(s, et, stream);
     }
     s->fix_keys += n_ord;
@@ -831,9 +903,13 @@ template <int KB>
 static int insert_impl(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream)
 {
     const KeyT<KB> *keys = (const KeyT<KB> *)d_keys;
-    uint64_t cap = s->p.chunk_keys ? s->p.chunk_keys : (KB == 4 ? (1ull << 28) : (1ull << 27));
-    if (cap > 0x7fffffffull)
-        cap = 0x7fffffffull;
+    // chunk limit: the stream index field of a packed item, and 2^30
+    uint64_t cap = s->p.chunk_keys ? s->p.chunk_keys : (1ull << 28);
+    const uint32_t ibits = 64u - s->pk.ishift;
+    if (ibits < 63u && cap > (1ull << ibits))
+        cap = 1ull << ibits;
+    if (cap > (1ull << 30))
+        cap = 1ull << 30;
     size_t done = 0;
     while (done < n) {
         uint64_t cn64 = n - done < cap ? n - done : cap;

@@ -98,6 +98,30 @@ __device__ static inline void qs_hash_index(uint64_t key, const qs_hashpar &hp, 
     mu = (uint32_t)(h % (uint64_t)hp.n_masks);
 }
 
+// One insert after hashing: bucket | mask index << bb | chunk-local stream
+// index << (bb + bm), in 64 bits.  Built once in the first radix pass, so no
+// later pass or kernel hashes again.
+struct qs_pack {
+    uint32_t bb;     // bucket bits
+    uint32_t bm;     // mask-index bits
+    uint32_t ishift; // bb + bm
+    uint32_t mmask;  // (1 << bm) - 1
+    uint64_t bmask;  // (1 << bb) - 1
+};
+
+__host__ __device__ static inline uint32_t it_b(uint64_t it, const qs_pack &pk)
+{
+    return (uint32_t)(it & pk.bmask);
+}
+__host__ __device__ static inline uint32_t it_mu(uint64_t it, const qs_pack &pk)
+{
+    return (uint32_t)(it >> pk.bb) & pk.mmask;
+}
+__host__ __device__ static inline uint32_t it_idx(uint64_t it, const qs_pack &pk)
+{
+    return (uint32_t)(it >> pk.ishift);
+}
+
 // Small device-resident control block of one sketch.
 struct qs_dev_state {
     uint32_t next;      // next high id (starts at 1025)
@@ -126,14 +150,14 @@ struct qs_sketch {
     qs_hashpar hp;
     uint64_t chunk;      // capacity (keys) of the chunk scratch
     int bucket_bits;     // bits of (n_low - 1)
+    qs_pack pk;          // layout of a packed (bucket, mask index, stream index) item
 
     uint32_t *d_low;
     uint32_t *d_high;
     uint32_t *d_masks;
     qs_dev_state *d_st;
     uint32_t *d_longbm;     // bitmap over buckets: segment handled by the tail kernel
-    void *d_k[2];           // chunk keys sorted by bucket (double buffer)
-    uint32_t *d_i[2];       // their chunk-local stream indices
+    uint64_t *d_it[2];      // packed chunk items sorted by bucket (double buffer)
     uint32_t *d_hist;       // radix histograms (shared by both sorts)
     uint32_t *d_ord;        // heads of segments for the in-order replay
     uint32_t *d_long;       // heads of long segments

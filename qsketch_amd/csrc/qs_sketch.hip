@@ -265,10 +265,20 @@ __device__ static inline uint32_t lds_mask(const uint32_t *s_masks, uint32_t mu)
 
 
 // ===========================================================================
-// Query: one lane per key, the 32-byte bucket as two 16-byte loads.
-// Replaces search_kernel (v3:410-521): est = min over the three selected byte
-// counters + (linked ? min over the same three high counters : 0).
+// Query: two lanes per key, each loading one 16-byte half of the 32-byte
+// bucket, so the pair issues a single coalesced request per key (a random
+// access costs one DRAM activation whatever its size, so requests per key are
+// what bounds this kernel).  Replaces search_kernel (v3:410-521):
+// est = min over the three selected byte counters
+//       + (linked ? min over the same three high counters : 0).
 // ===========================================================================
+__device__ static inline uint32_t sel4(const uint4 &v, uint32_t k)
+{
+    uint32_t lo = (k & 1u) ? v.y : v.x;
+    uint32_t hi = (k & 1u) ? v.w : v.z;
+    return (k & 2u) ? hi : lo;
+}
+
 template <int KB>
 __global__ __launch_bounds__(INS_THREADS) void k_query(const KeyT<KB> *__restrict__ keys, uint32_t n,
                                                        const uint32_t *__restrict__ low,
@@ -280,29 +290,31 @@ __global__ __launch_bounds__(INS_THREADS) void k_query(const KeyT<KB> *__restric
     for (uint32_t i = threadIdx.x; i < hp.n_masks; i += INS_THREADS)
         s_masks[i] = masks[i];
     __syncthreads();
-    const uint32_t stride = gridDim.x * INS_THREADS;
-    for (uint32_t i = blockIdx.x * INS_THREADS + threadIdx.x; i < n; i += stride) {
-        uint32_t b, mu;
-        qs_hash_index(keys[i], hp, b, mu);
-        uint32_t M = lds_mask(s_masks, mu);
-        const uint4 *bp = reinterpret_cast<const uint4 *>(low + (size_t)b * 8u);
-        uint4 a = bp[0];
-        uint4 c = bp[1];
-        uint32_t j1 = (uint32_t)__builtin_ctz(M);
-        uint32_t M1 = M & (M - 1u);
-        uint32_t j2 = (uint32_t)__builtin_ctz(M1);
-        uint32_t M2 = M1 & (M1 - 1u);
-        uint32_t j3 = (uint32_t)__builtin_ctz(M2);
-        uint32_t c1 = byte_at(sel_word(a, c, j1 >> 2), j1);
-        uint32_t c2 = byte_at(sel_word(a, c, j2 >> 2), j2);
-        uint32_t c3 = byte_at(sel_word(a, c, j3 >> 2), j3);
-        uint32_t est = min(c1, min(c2, c3));
-        uint32_t w = c.w;
-        if (w > QS_LINK_START) {
-            const uint32_t *g = high + (size_t)(w - QS_LINK_START) * 32u;
-            est += min(g[j1], min(g[j2], g[j3]));
+    const uint32_t half = threadIdx.x & 1u;
+    const uint64_t stride = (uint64_t)gridDim.x * (INS_THREADS / 2);
+    for (uint64_t i = (uint64_t)blockIdx.x * (INS_THREADS / 2) + (threadIdx.x >> 1); i < n; i += stride) {
+        const KeyT<KB> key = keys[i];
+        const uint32_t M = s_masks[qs_maskidx(key, hp)];
+        const uint4 v = reinterpret_cast<const uint4 *>(low + (size_t)qs_bucket(key, hp) * 8u)[half];
+        const uint32_t Mh = half ? (M >> 16) : (M & 0xffffu);
+        uint32_t lo = 0xffffffffu;
+        for (uint32_t m = Mh; m; m &= m - 1u) {
+            const uint32_t jj = (uint32_t)__builtin_ctz(m);
+            lo = min(lo, (sel4(v, jj >> 2) >> ((jj & 3u) * 8u)) & 0xffu);
         }
-        out[i] = est;
+        lo = min(lo, (uint32_t)__shfl_xor(lo, 1, 64));
+        const uint32_t wo = (uint32_t)__shfl_xor(v.w, 1, 64);
+        const uint32_t w = half ? v.w : wo;
+        uint32_t hi = 0;
+        if (w > QS_LINK_START) {
+            const uint32_t *g = high + (size_t)(w - QS_LINK_START) * 32u + 16u * half;
+            hi = 0xffffffffu;
+            for (uint32_t m = Mh; m; m &= m - 1u)
+                hi = min(hi, g[__builtin_ctz(m)]);
+            hi = min(hi, (uint32_t)__shfl_xor(hi, 1, 64));
+        }
+        if (!half)
+            out[i] = lo + hi;
     }
 }
 
@@ -424,6 +436,11 @@ extern "C" int qs_create(const qs_params *p, qs_sketch **out)
     size_t low_bytes = (size_t)p->n_low * 32u, high_bytes = (size_t)p->n_high * 128u;
     size_t viol_words = (size_t)((p->n_low + 31) / 32);
     s->bucket_bits = p->n_low > 1 ? 64 - __builtin_clzll(p->n_low - 1) : 0;
+    s->pk.bb = s->bucket_bits > 0 ? (uint32_t)s->bucket_bits : 1u;
+    s->pk.bm = p->n_masks > 1 ? 32u - (uint32_t)__builtin_clz(p->n_masks - 1u) : 1u;
+    s->pk.ishift = s->pk.bb + s->pk.bm;
+    s->pk.mmask = (1u << s->pk.bm) - 1u;
+    s->pk.bmask = (1ull << s->pk.bb) - 1ull;
     bool ok = hipMalloc(&s->d_low, low_bytes) == hipSuccess &&
               hipMalloc(&s->d_high, high_bytes) == hipSuccess &&
               hipMalloc(&s->d_masks, sizeof(uint32_t) * p->n_masks) == hipSuccess &&
@@ -512,11 +529,11 @@ static int query_dispatch(qs_sketch *s, const void *d_keys, size_t n, uint32_t *
     qs_evpair ev;
     qs_tim_begin(s, ev, QS_TK_QUERY, nn, stream);
     if (s->p.key_bytes == 4)
-        hipLaunchKernelGGL(k_query<4>, dim3(grid_for(nn, INS_THREADS, 8)), dim3(INS_THREADS), 0, stream,
+        hipLaunchKernelGGL(k_query<4>, dim3(grid_for((uint64_t)nn * 2, INS_THREADS, 8)), dim3(INS_THREADS), 0, stream,
                            (const uint32_t *)d_keys, nn, s->d_low, s->d_high, s->d_masks, s->hp,
                            d_counts);
     else
-        hipLaunchKernelGGL(k_query<8>, dim3(grid_for(nn, INS_THREADS, 8)), dim3(INS_THREADS), 0, stream,
+        hipLaunchKernelGGL(k_query<8>, dim3(grid_for((uint64_t)nn * 2, INS_THREADS, 8)), dim3(INS_THREADS), 0, stream,
                            (const uint64_t *)d_keys, nn, s->d_low, s->d_high, s->d_masks, s->hp,
                            d_counts);
     qs_tim_end(s, ev, stream);
