@@ -57,5 +57,24 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_native_tests(verbose: bool = False) -> None:
+    """C++ drivers used by the GPU tests: the include/qsketch.hpp interface
+    against the oracle, and the exhaustive fastmod check."""
+    root = os.path.dirname(HERE)
+    nat = os.path.join(root, "tests", "native")
+    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           os.path.join(nat, "cpp_api_test.cpp"), "-L" + HERE, "-lqsketch_amd",
+           "-L" + os.path.join(root, "oracle"), "-loracle",
+           "-Wl,-rpath,$ORIGIN/../../qsketch_amd", "-Wl,-rpath,$ORIGIN/../../oracle",
+           "-o", os.path.join(nat, "cpp_api_test")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    cmd = [HIPCC, "-O2", "-std=c++17", os.path.join(nat, "fastmod_check.cpp"),
+           "-o", os.path.join(nat, "fastmod_check")]
+    _run(cmd)
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -1,0 +1,139 @@
+"""CPU-side checks of the C ABI library (no GPU compute): it loads, exports
+every entry point the public headers declare, and its host-only functions
+(sizing, primes, error strings, workload generators) behave."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
+
+
+def _declared(path):
+    src = open(path).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b(qs_\w+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    import qsketch_amd as q
+    L = q.lib()
+    declared = [n for h in HEADERS for n in _declared(h)]
+    assert len(declared) >= 20
+    missing = [n for n in declared if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding knows every one of them
+    assert not [n for n in declared if n not in q._SIGS], [n for n in declared if n not in q._SIGS]
+
+
+def test_default_params_follow_reference_defaults():
+    import qsketch_amd as q
+    p = q.default_params(1 << 22, 0.5)
+    assert (p.m_low, p.m_high, p.n_high, p.seed_sz, p.n_masks) == (8, 32, 1024, 1, 1021)
+    assert (p.mask_ones, p.link_start, p.low_limit, p.key_bytes) == (3, 1024, 128, 4)
+    assert p.n_low == 3145721
+    assert q.default_params(1 << 28, 1.0).n_low == 100663291
+    assert q.default_params(1 << 28, 0.5).n_low == 201326557
+
+
+def test_primes_match_oracle():
+    import qsketch_amd as q
+    from oracle import pyoracle as po
+    for x in [0, 1, 2, 3, 100, 1021, 1022, 4095, 4096, 123456789, 1 << 28, 1 << 33]:
+        assert q.find_greatest_prime(x) == po.find_greatest_prime(x)
+    for k in range(10, 30):
+        assert q.number_of_buckets(1 << k, 8, 0.5) == po.number_of_buckets(1 << k, 8, 0.5)
+        assert q.number_of_buckets(1 << k, 8, 1.0) == po.number_of_buckets(1 << k, 8, 1.0)
+
+
+def test_error_strings_and_bad_params():
+    import qsketch_amd as q
+    L = q.lib()
+    for code in (0, -1, -2, -3, -4, -5, -6, -99):
+        assert L.qs_strerror(code)
+    p = q.default_params(1 << 20)
+    h = C.c_void_p()
+    for field, bad in (("m_low", 4), ("m_high", 16), ("mask_ones", 2), ("link_start", 7),
+                       ("low_limit", 100), ("key_bytes", 2), ("n_high", 1), ("n_masks", 0),
+                       ("n_masks", 5000), ("n_low", 0), ("semantics", 7), ("seed_sz", 0)):
+        pp = q.qs_params.from_buffer_copy(p)
+        setattr(pp, field, bad)
+        assert L.qs_create(C.byref(pp), C.byref(h)) == q.QS_ERR_PARAM, field
+    pp = q.qs_params.from_buffer_copy(p)
+    pp.n_low = (1 << 29) + 1  # 32-bit keys: the word offset must fit 32 bits (v3:42)
+    assert L.qs_create(C.byref(pp), C.byref(h)) == q.QS_ERR_PARAM
+    assert L.qs_default_params(C.byref(pp), 1 << 20, 0.0) == q.QS_ERR_PARAM
+    assert L.qs_insert(None, None, 0) == q.QS_ERR_PARAM
+    assert L.qs_query(None, None, 0, None) == q.QS_ERR_PARAM
+
+
+def test_create_without_gpu_reports_no_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    import qsketch_amd as q
+    p = q.default_params(1 << 20)
+    h = C.c_void_p()
+    assert q.lib().qs_create(C.byref(p), C.byref(h)) == q.QS_ERR_NODEVICE
+    assert not h.value
+
+
+# ---------------------------------------------------------------------------
+# workload generators (host versions; the GPU versions share the sampler)
+# ---------------------------------------------------------------------------
+def test_zipf_sampler_matches_distribution():
+    import qsketch_amd as q
+    U = 1 << 12
+    for alpha in (0.8, 1.1, 1.5):
+        z = q.Zipf(alpha, U, device=False)
+        r = z.ranks_host(7, 1 << 20)
+        assert r.max() < U
+        p = (np.arange(1, U + 1, dtype=np.float64) ** -alpha)
+        p /= p.sum()
+        obs = np.bincount(r, minlength=U)
+        exp = p * r.size
+        top = 64
+        chi = ((obs[:top] - exp[:top]) ** 2 / exp[:top]).sum()
+        assert chi < 3 * top, (alpha, chi)
+        assert abs(obs[top:].sum() - exp[top:].sum()) < 6 * np.sqrt(exp[top:].sum())
+
+
+def test_generators_are_deterministic_and_offsettable():
+    import qsketch_amd as q
+    z = q.Zipf(1.1, 1 << 16, device=False)
+    a = z.keys_u32_host(3, 5000)
+    b = z.keys_u32_host(3, 5000)
+    assert np.array_equal(a, b)
+    assert np.array_equal(z.keys_u32_host(3, 1000, offset=4000), a[4000:])
+    assert not np.array_equal(z.keys_u32_host(4, 5000), a)
+    u = q.uniform_u32_host(9, 1 << 16)
+    assert np.array_equal(q.uniform_u32_host(9, 100, offset=50), u[50:150])
+    assert abs(u.astype(np.float64).mean() / 2 ** 32 - 0.5) < 0.01
+
+
+def test_rank_to_key_is_a_bijection_on_a_range():
+    import qsketch_amd as q
+    ks = np.array([q.rank_to_key32(5, r) for r in range(1 << 14)], dtype=np.uint64)
+    assert np.unique(ks).size == ks.size
+
+
+def test_noisy_u64_keys_encode_log_rank():
+    import qsketch_amd as q
+    z = q.Zipf(1.1, 1 << 16, device=False)
+    k = z.keys_u64_noisy_host(11, 1 << 16, sigma=1.5)
+    hi = (k >> np.uint64(48)).astype(np.int64)
+    assert hi.min() >= 0 and hi.max() <= 31
+    r = z.ranks_host(11, 1 << 16).astype(np.float64)
+    base = np.clip(np.round(np.log2(r + 1)), 0, 31)
+    # noise has mean ~0 and sd ~1.5 before clipping
+    d = hi - base
+    assert abs(d.mean()) < 0.5 and 0.8 < d[(base > 4) & (base < 26)].std() < 2.2
+    # the same rank always maps to the same key
+    assert np.unique(k).size == np.unique(z.ranks_host(11, 1 << 16)).size
