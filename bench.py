@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24,
                     help="log2 of the stream prefix the CPU baseline inserts")
+    ap.add_argument("--workload", choices=["plain", "multi"], default="plain",
+                    help="plain: one sketch (default); multi: 4096 independent sketches in one batch")
+    ap.add_argument("--sketches", type=int, default=4096, help="multi: number of sketches")
+    ap.add_argument("--no-tune", action="store_true", help="multi: skip the shape autotuner")
     return ap.parse_args()
 
 
@@ -77,6 +81,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "designed (see DESIGN.md) but not built", "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.workload == "multi":
+        return bench_multi(a)
     import torch
     import qsketch_amd as q
 
@@ -166,11 +172,146 @@ def main():
             "bytes_per_key": QUERY_BYTES_PER_KEY,
             "avg_launch_ms": round(q_ms / max(1, tim["query_kernel_launches"]), 4),
         },
-        "fixup": {"segments_replayed_per_step": st["fixup_keys"] // max(1, a.steps + a.warmup),
+        "fixup": {"keys_replayed_per_step": st["fixup_keys"],
+                  "segments_replayed_per_step": st["ordered_segments"],
+                  "long_segments_per_step": st["long_segments"],
                   "linked_buckets": st["linked_buckets"]},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(keys, n_low, a.n_high, a.cpu_sample_log2)
+    print(json.dumps(out), flush=True)
+
+
+def multi_workload(n_sk, seed=1):
+    """4096 independent streams: lengths log-uniform in [2^10, 2^20], each Zipf
+    with its own exponent (uniform in [0.7, 1.3], on a 0.01 grid so that one
+    sampler table serves many streams), its own seed and its own 2^16-key
+    universe (the rank -> key permutation is keyed by the stream's seed)."""
+    import numpy as np
+    rs = np.random.default_rng(seed)
+    lengths = np.rint(2.0 ** rs.uniform(10, 20, n_sk)).astype(np.int64)
+    alphas = np.round(rs.uniform(0.7, 1.3, n_sk), 2)
+    seeds = np.arange(1, n_sk + 1, dtype=np.int64) * 7919 + seed
+    return lengths, alphas, seeds
+
+
+def bench_multi(a):
+    """Batched multi-sketch: one captured graph launch inserts every stream into
+    its own sketch, against the same streams inserted by one plain sketch each."""
+    import numpy as np
+    import torch
+    import qsketch_amd as q
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    K = a.sketches
+    lengths, alphas, seeds = multi_workload(K)
+    offs = np.zeros(K + 1, np.uint64)
+    offs[1:] = np.cumsum(lengths)
+    N = int(offs[-1])
+    keys = torch.empty(N, dtype=torch.int32, device=dev)
+    samplers = {}
+    for i in range(K):
+        al = float(alphas[i])
+        if al not in samplers:
+            samplers[al] = q.Zipf(al, 1 << 16)
+        samplers[al].keys_u32(int(seeds[i]), int(lengths[i]), out=keys[int(offs[i]):int(offs[i + 1])])
+    d_offs = torch.from_numpy(offs.view(np.int64).copy()).to(dev)
+    params = []
+    for i in range(K):
+        n_i = int(lengths[i])
+        params.append(q.make_params(q.number_of_buckets(n_i, 8, 0.5), max(1024, n_i // 256),
+                                    rng_seed=int(seeds[i])))
+    S = q.SketchSet(params)
+    tune = None
+    if not a.no_tune:
+        tune = S.autotune(keys, offs, d_offs, reps=1)
+    g = S.capture_insert(keys, offs, d_offs)
+    torch.cuda.synchronize()
+
+    def run_set():
+        S.clear()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.launch(0)
+        S.sync()
+        return time.perf_counter() - t0
+
+    for _ in range(a.warmup):
+        run_set()
+    t_set = [run_set() for _ in range(a.steps)]
+    S.set_timing(True)
+    run_set()
+    tim = S.timing()
+    S.set_timing(False)
+
+    # the same streams, one plain sketch and one blocking insert call each
+    plain = [q.Sketch_GPU_Sub_Warp_Level(int(p.n_low), 8, int(p.n_high), 32, rng_seed=int(p.rng_seed))
+             for p in params]
+    views = [keys[int(offs[i]):int(offs[i + 1])] for i in range(K)]
+
+    def run_seq(blocking):
+        for sk in plain:
+            sk.clear()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for sk, v in zip(plain, views):
+            if blocking:
+                sk.insert(v)
+            else:
+                sk.insert_async(v, v.numel(), 0)
+        torch.cuda.synchronize()
+        for sk in plain:
+            sk.sync()
+        return time.perf_counter() - t0
+
+    run_seq(True)
+    t_seq = min(run_seq(True) for _ in range(max(1, a.steps)))
+    t_seq_async = min(run_seq(False) for _ in range(max(1, a.steps)))
+
+    # parity: every sketch of the set gives the plain sketch's estimates on all
+    # distinct keys of its stream
+    uq = [torch.unique(v) for v in views]
+    qoffs = np.zeros(K + 1, np.uint64)
+    qoffs[1:] = np.cumsum([u.numel() for u in uq])
+    qkeys = torch.cat(uq)
+    est_set = S.search(qkeys, qoffs)
+    est_plain = torch.cat([sk.search(u) for sk, u in zip(plain, uq)])
+    identical = bool(torch.equal(est_set, est_plain))
+    st = S.stats()
+
+    dt = float(np.median(t_set))
+    out = {
+        "metric": "batched multi-sketch insert Gkeys/s (one captured launch) vs sequential plain inserts",
+        "value": round(N / dt / 1e9, 4),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{K} sketches, stream lengths log-uniform in [2^10, 2^20] u32 keys, Zipf alpha "
+                        f"uniform in [0.7, 1.3] (0.01 grid), own seed and 2^16-key universe each; "
+                        f"n_low per sketch by the sizing rule at WLF 0.5, n_high max(1024, n/256)",
+            "keys": N, "sketches": K, "memory_bytes": S.memory_bytes(),
+        },
+        "sequential_blocking": {"gkeys_s": round(N / t_seq / 1e9, 4), "ms": round(t_seq * 1e3, 3),
+                                "calls": K},
+        "sequential_async": {"gkeys_s": round(N / t_seq_async / 1e9, 4), "ms": round(t_seq_async * 1e3, 3)},
+        "speedup_vs_sequential_blocking": round(t_seq / dt, 2),
+        "estimates_identical_to_plain_sketches": identical,
+        "distinct_keys_checked": int(qoffs[-1]),
+        "shape": list(S.shape()),
+        "tuner": tune,
+        "kernel_ms_per_step": {k: round(v["ms"], 4) for k, v in tim["kernels"].items() if v["launches"]},
+        "linked_buckets": st["linked_buckets"],
+    }
+    g.destroy()
     print(json.dumps(out), flush=True)
 
 

@@ -64,7 +64,7 @@ typedef struct qs_params {
     int32_t  semantics;  /* QS_LITERAL (default) or QS_PAPER                      */
     uint32_t hash_seed;  /* 0: draw the seeds from rng_seed; else use this value   */
     uint64_t rng_seed;   /* replaces the reference's /dev/urandom engine seeding (tool.h:192-201) */
-    uint64_t chunk_keys; /* keys per internal insert chunk (0 = default 2^24)      */
+    uint64_t chunk_keys; /* keys per internal insert chunk (0 = default 2^28, capped by the packing) */
 } qs_params;
 
 typedef struct qs_sketch qs_sketch;
@@ -123,18 +123,24 @@ int qs_export_state(qs_sketch *s, uint32_t *low, uint32_t *high, uint32_t *masks
 int qs_import_state(qs_sketch *s, const uint32_t *low, const uint32_t *high,
                     const uint32_t *masks, uint32_t seed, uint32_t next);
 
-/* ---- statistics (build addition) ----------------------------------------- */
+/* ---- statistics (build addition) -----------------------------------------
+ * For a set of sketches (qs_multi_create) the sizes and linked_buckets are
+ * sums over the set; next_id and seed are sketch 0's.                       */
 typedef struct qs_stats {
     uint64_t n_low, n_high, n_masks;
     uint64_t bytes_low, bytes_high;     /* table sizes, cf. show_memory_usage
                                            (device_sketch.h:403-407) */
     uint64_t keys_inserted;
     uint64_t chunks;                    /* internal insert chunks processed */
-    uint64_t chunks_with_fixup;         /* chunks that needed the ordered fix-up */
-    uint64_t fixup_keys;                /* keys replayed in stream order */
+    uint64_t chunks_with_fixup;         /* chunks with at least one in-order replay */
+    uint64_t fixup_keys;                /* inserts replayed in stream order */
     uint64_t linked_buckets;            /* high buckets in use (= next - 1025) */
     uint32_t next_id;
     uint32_t seed;
+    uint64_t ordered_segments;          /* bucket segments sent to the in-order replay */
+    uint64_t long_segments;             /* bucket segments handled by the tail pass */
+    uint32_t n_sketches;
+    uint32_t pool_error;                /* 1 once a high pool overflowed */
 } qs_stats;
 int qs_get_stats(qs_sketch *s, qs_stats *st);
 
@@ -167,6 +173,75 @@ int qs_set_timing(qs_sketch *s, int enable);
 int qs_get_timing(qs_sketch *s, qs_timing *t);
 
 const char *qs_strerror(int code);
+
+/* ---- batched multi-sketch (build addition) --------------------------------
+ * The reference runs one sketch per object and one launch per insert call.
+ * A "set" holds n independent sketches (each with its own parameters, seeds,
+ * masks and high pool; all with the same key width) and updates all of them
+ * with one launch sequence per batch: a batch is one key array in which
+ * sketch i owns keys[offs[i] .. offs[i+1]) (offs has n + 1 entries, offs[0] =
+ * 0).  Every sketch of a set ends in exactly the state the plain sketch with
+ * the same parameters reaches on its own stream.  The plain entry points
+ * (qs_insert, qs_query, qs_export_state, qs_import_state) take sets of one
+ * only; qs_clear, qs_destroy, qs_sync, qs_get_stats and timing take any set.
+ * Calls on one set must be ordered (one stream at a time).                  */
+int qs_multi_create(const qs_params *ps, uint32_t n_sketches, qs_sketch **out);
+uint32_t qs_multi_count(const qs_sketch *s);
+int qs_multi_params(const qs_sketch *s, uint32_t sketch, qs_params *p);
+/* blocking; h_offs is a host array */
+int qs_multi_insert(qs_sketch *s, const void *d_keys, const uint64_t *h_offs);
+int qs_multi_query(qs_sketch *s, const void *d_keys, const uint64_t *h_offs, uint32_t *d_counts);
+/* stream-ordered: h_offs (host) plans the chunks, d_offs (device, n + 1
+ * entries, same values) is read by the kernels and must stay valid until the
+ * work has run; pool errors surface at qs_sync */
+int qs_multi_insert_async(qs_sketch *s, const void *d_keys, const uint64_t *h_offs,
+                          const uint64_t *d_offs, void *stream);
+int qs_multi_query_async(qs_sketch *s, const void *d_keys, const uint64_t *d_offs, uint64_t n_keys,
+                         uint32_t *d_counts, void *stream);
+int qs_multi_export_state(qs_sketch *s, uint32_t sketch, uint32_t *low, uint32_t *high,
+                          uint32_t *masks, uint32_t *seed, uint32_t *next);
+int qs_multi_import_state(qs_sketch *s, uint32_t sketch, const uint32_t *low, const uint32_t *high,
+                          const uint32_t *masks, uint32_t seed, uint32_t next);
+
+/* Captured insert: the launch sequence of one insert batch (any set, a set of
+ * one included) recorded once into a hipGraph and replayed with one call.  The
+ * graph reads d_keys and d_offs at replay time, so the caller may refill them
+ * between replays with the same layout h_offs.  Destroy graphs before their
+ * set; while a graph lives, import is refused (QS_ERR_STATE: the graph holds
+ * the hash parameters) and so is a later batch that needs larger scratch
+ * buffers than the graph's.  Build addition with no reference counterpart. */
+typedef struct qs_graph qs_graph;
+int qs_graph_capture_insert(qs_sketch *s, const void *d_keys, const uint64_t *h_offs,
+                            const uint64_t *d_offs, qs_graph **out);
+int qs_graph_launch(qs_graph *g, void *stream);
+void qs_graph_destroy(qs_graph *g);
+
+/* ---- insert shape and autotuner (build addition) --------------------------
+ * The per-bucket-segment pass of the insert runs in groups of `width` lanes
+ * (a sub-wavefront: 8, 16, 32 or 64) over `items_per_lane` (1, 2, 4 or 8)
+ * consecutive sorted inserts per lane; (0, 0) selects the default 64-lane,
+ * one-item kernel.  The shape never changes the result.  The autotuner runs
+ * one insert batch with all 16 shapes on a twin set (the caller's set is not
+ * modified; the twin needs as much device memory as the set), checks that all
+ * of them leave the same state, and keeps the shape with the fastest segment
+ * pass on the set (QS_ERR_STATE, shape unchanged, if two shapes disagree).   */
+typedef struct qs_tune_entry {
+    uint32_t width, items_per_lane;
+    double seg_ms;     /* segment pass, mean over reps */
+    double insert_ms;  /* whole insert launch sequence, mean over reps */
+    uint64_t checksum; /* state checksum after the insert */
+} qs_tune_entry;
+typedef struct qs_tune_result {
+    uint32_t n;          /* entries measured */
+    uint32_t best;       /* index of the chosen entry */
+    uint32_t consistent; /* 1 if every shape left the same state */
+    uint32_t pad;
+    qs_tune_entry e[16];
+} qs_tune_result;
+int qs_set_insert_shape(qs_sketch *s, uint32_t width, uint32_t items_per_lane);
+int qs_get_insert_shape(const qs_sketch *s, uint32_t *width, uint32_t *items_per_lane);
+int qs_autotune_insert(qs_sketch *s, const void *d_keys, const uint64_t *h_offs, const uint64_t *d_offs,
+                       int reps, qs_tune_result *out);
 
 #ifdef __cplusplus
 }

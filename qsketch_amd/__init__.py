@@ -58,6 +58,8 @@ class qs_stats(C.Structure):
         ("bytes_low", C.c_uint64), ("bytes_high", C.c_uint64), ("keys_inserted", C.c_uint64),
         ("chunks", C.c_uint64), ("chunks_with_fixup", C.c_uint64), ("fixup_keys", C.c_uint64),
         ("linked_buckets", C.c_uint64), ("next_id", C.c_uint32), ("seed", C.c_uint32),
+        ("ordered_segments", C.c_uint64), ("long_segments", C.c_uint64),
+        ("n_sketches", C.c_uint32), ("pool_error", C.c_uint32),
     ]
 
 
@@ -71,6 +73,16 @@ class qs_timing(C.Structure):
         ("query_kernel_launches", C.c_uint64), ("query_keys", C.c_uint64),
         ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8),
     ]
+
+
+class qs_tune_entry(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("items_per_lane", C.c_uint32), ("seg_ms", C.c_double),
+                ("insert_ms", C.c_double), ("checksum", C.c_uint64)]
+
+
+class qs_tune_result(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("best", C.c_uint32), ("consistent", C.c_uint32), ("pad", C.c_uint32),
+                ("e", qs_tune_entry * 16)]
 
 
 # every symbol declared in include/qsketch.h and include/qsketch_gen.h etc.
@@ -91,6 +103,26 @@ _SIGS = {
     "qs_strerror": (C.c_char_p, [C.c_int]),
     "qs_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "qs_get_timing": (C.c_int, [C.c_void_p, C.POINTER(qs_timing)]),
+    # sets of sketches and captured inserts
+    "qs_multi_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "qs_multi_count": (C.c_uint32, [C.c_void_p]),
+    "qs_multi_params": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(qs_params)]),
+    "qs_multi_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qs_multi_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qs_multi_insert_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "qs_multi_query_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                       C.c_void_p]),
+    "qs_multi_export_state": (C.c_int, [C.c_void_p, C.c_uint32] + [C.c_void_p] * 5),
+    "qs_multi_import_state": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_uint32, C.c_uint32]),
+    "qs_graph_capture_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.POINTER(C.c_void_p)]),
+    "qs_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "qs_graph_destroy": (None, [C.c_void_p]),
+    "qs_set_insert_shape": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "qs_get_insert_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "qs_autotune_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.POINTER(qs_tune_result)]),
     # generators (include/qsketch_gen.h)
     "qs_zipf_create": (C.c_int, [C.c_double, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
     "qs_zipf_destroy": (None, [C.c_void_p]),
@@ -235,6 +267,16 @@ class Sketch_GPU_Sub_Warp_Level:
     def sync(self) -> None:
         check(_lib().qs_sync(self._h), "qs_sync")
 
+    def capture_insert(self, keys, n: int | None = None) -> "Graph":
+        """Record the insert of n keys at this address into a hipGraph."""
+        import numpy as np
+        n = keys.numel() if n is None else int(n)
+        o = np.array([0, n], dtype=np.uint64)
+        g = C.c_void_p()
+        check(_lib().qs_graph_capture_insert(self._h, _ptr(keys), o.ctypes.data, None, C.byref(g)),
+              "qs_graph_capture_insert")
+        return Graph(g, self)
+
     def export_state(self):
         import numpy as np
         p = self.params
@@ -283,6 +325,168 @@ class Sketch_GPU_Sub_Warp_Level:
 
 # convenience alias
 QSketch = Sketch_GPU_Sub_Warp_Level
+
+
+def make_params(n_low: int, n_high: int = 1024, *, n_masks: int = HASH_MASK_TABLE_SZ, seed_sz: int = SEED_SZ,
+                key_bytes: int = 4, semantics: int = QS_LITERAL, rng_seed: int = 1, hash_seed: int = 0,
+                chunk_keys: int = 0) -> qs_params:
+    """A parameter set with the reference's fixed fields (m_low 8, m_high 32, ...)."""
+    p = qs_params()
+    _lib().qs_default_params(C.byref(p), 1 << 20, 0.5)
+    p.n_low, p.n_high, p.n_masks, p.seed_sz = int(n_low), int(n_high), int(n_masks), int(seed_sz)
+    p.key_bytes, p.semantics, p.rng_seed = int(key_bytes), int(semantics), int(rng_seed)
+    p.hash_seed, p.chunk_keys = int(hash_seed) & 0xffffffff, int(chunk_keys)
+    return p
+
+
+def _offs_array(offs):
+    import numpy as np
+    o = np.ascontiguousarray(offs, dtype=np.uint64)
+    if o.ndim != 1 or o.size < 2 or o[0] != 0 or np.any(np.diff(o.astype(np.int64)) < 0):
+        raise ValueError("offsets must start at 0 and be non-decreasing")
+    return o
+
+
+class SketchSet:
+    """A set of independent MSWS sketches updated together (include/qsketch.h,
+    qs_multi_*): sketch i owns keys[offs[i]:offs[i+1]] of every batch and ends in
+    the state the plain sketch with its parameters reaches on its own stream."""
+
+    def __init__(self, params):
+        arr = (qs_params * len(params))(*params)
+        self.params = list(arr)
+        self.n = len(params)
+        self.key_bytes = int(params[0].key_bytes)
+        h = C.c_void_p()
+        check(_lib().qs_multi_create(arr, self.n, C.byref(h)), "qs_multi_create")
+        self._h = h
+
+    def insert(self, keys, offs) -> None:
+        o = _offs_array(offs)
+        check(_lib().qs_multi_insert(self._h, _ptr(keys), o.ctypes.data), "qs_multi_insert")
+
+    def insert_async(self, keys, offs, d_offs, stream: int = 0) -> None:
+        o = _offs_array(offs)
+        check(_lib().qs_multi_insert_async(self._h, _ptr(keys), o.ctypes.data, _ptr(d_offs),
+                                           C.c_void_p(stream)), "qs_multi_insert_async")
+
+    def search(self, keys, offs, counts=None):
+        import torch
+        o = _offs_array(offs)
+        if counts is None:
+            counts = torch.empty(int(o[-1]), dtype=torch.int32, device=keys.device)
+        check(_lib().qs_multi_query(self._h, _ptr(keys), o.ctypes.data, _ptr(counts)), "qs_multi_query")
+        return counts
+
+    query = search
+
+    def search_async(self, keys, d_offs, n: int, counts, stream: int = 0) -> None:
+        check(_lib().qs_multi_query_async(self._h, _ptr(keys), _ptr(d_offs), int(n), _ptr(counts),
+                                          C.c_void_p(stream)), "qs_multi_query_async")
+
+    def capture_insert(self, keys, offs, d_offs) -> "Graph":
+        """Record the insert of one batch layout into a hipGraph (see Graph)."""
+        o = _offs_array(offs)
+        g = C.c_void_p()
+        check(_lib().qs_graph_capture_insert(self._h, _ptr(keys), o.ctypes.data, _ptr(d_offs), C.byref(g)),
+              "qs_graph_capture_insert")
+        return Graph(g, self)
+
+    def clear(self) -> None:
+        check(_lib().qs_clear(self._h), "qs_clear")
+
+    def sync(self) -> None:
+        check(_lib().qs_sync(self._h), "qs_sync")
+
+    def set_shape(self, width: int, items_per_lane: int) -> None:
+        """Segment-pass shape: sub-wavefront width x items per lane ((0, 0) = default)."""
+        check(_lib().qs_set_insert_shape(self._h, int(width), int(items_per_lane)), "qs_set_insert_shape")
+
+    def shape(self):
+        w, i = C.c_uint32(), C.c_uint32()
+        check(_lib().qs_get_insert_shape(self._h, C.byref(w), C.byref(i)), "qs_get_insert_shape")
+        return w.value, i.value
+
+    def autotune(self, keys, offs, d_offs, reps: int = 2) -> dict:
+        """Time all 16 shapes on a twin set with this batch; keep the fastest."""
+        o = _offs_array(offs)
+        r = qs_tune_result()
+        rc = _lib().qs_autotune_insert(self._h, _ptr(keys), o.ctypes.data, _ptr(d_offs), int(reps), C.byref(r))
+        table = [{"width": r.e[i].width, "items_per_lane": r.e[i].items_per_lane,
+                  "seg_ms": round(r.e[i].seg_ms, 4), "insert_ms": round(r.e[i].insert_ms, 4),
+                  "checksum": r.e[i].checksum} for i in range(r.n)]
+        check(rc, "qs_autotune_insert")
+        return {"best": table[r.best], "consistent": bool(r.consistent), "table": table}
+
+    def export_state(self, i: int):
+        import numpy as np
+        p = self.params[i]
+        low = np.zeros(p.n_low * 8, dtype=np.uint32)
+        high = np.zeros(p.n_high * 32, dtype=np.uint32)
+        masks = np.zeros(p.n_masks, dtype=np.uint32)
+        seed = C.c_uint32()
+        nxt = C.c_uint32()
+        check(_lib().qs_multi_export_state(self._h, int(i), low.ctypes.data, high.ctypes.data,
+                                           masks.ctypes.data, C.addressof(seed), C.addressof(nxt)),
+              "qs_multi_export_state")
+        return {"low": low, "high": high, "masks": masks, "seed": seed.value, "next": nxt.value}
+
+    def stats(self) -> dict:
+        st = qs_stats()
+        check(_lib().qs_get_stats(self._h, C.byref(st)), "qs_get_stats")
+        return {f: getattr(st, f) for f, _ in qs_stats._fields_}
+
+    def set_timing(self, enable: bool = True) -> None:
+        check(_lib().qs_set_timing(self._h, 1 if enable else 0), "qs_set_timing")
+
+    def timing(self) -> dict:
+        t = qs_timing()
+        check(_lib().qs_get_timing(self._h, C.byref(t)), "qs_get_timing")
+        d = {f: getattr(t, f) for f, _ in qs_timing._fields_[:6]}
+        d["kernels"] = {k: {"ms": t.kernel_ms[i], "launches": t.kernel_launches[i]}
+                        for i, k in enumerate(TIMING_KERNELS)}
+        return d
+
+    def memory_bytes(self) -> int:
+        return sum(int(p.n_low) * 32 + int(p.n_high) * 128 for p in self.params)
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().qs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured insert (qs_graph_capture_insert).  Replays read the keys and
+    device offsets captured, so refill them in place between launches."""
+
+    def __init__(self, h, owner):
+        self._h = h
+        self._owner = owner  # keeps the sketch alive while the graph is
+
+    def launch(self, stream: int = 0) -> None:
+        check(_lib().qs_graph_launch(self._h, C.c_void_p(stream)), "qs_graph_launch")
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().qs_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class Zipf:

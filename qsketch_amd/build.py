@@ -59,7 +59,7 @@ def build(verbose: bool = False) -> str:
 
 def build_native_tests(verbose: bool = False) -> None:
     """C++ drivers used by the GPU tests: the include/qsketch.hpp interface
-    against the oracle, and the exhaustive fastmod check."""
+    against the oracle, the exhaustive fastmod check and the item packing check."""
     root = os.path.dirname(HERE)
     nat = os.path.join(root, "tests", "native")
     cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}",
@@ -73,6 +73,9 @@ def build_native_tests(verbose: bool = False) -> None:
     _run(cmd)
     cmd = [HIPCC, "-O2", "-std=c++17", os.path.join(nat, "fastmod_check.cpp"),
            "-o", os.path.join(nat, "fastmod_check")]
+    _run(cmd)
+    cmd = [HIPCC, "-O2", "-std=c++17", os.path.join(nat, "pack_check.cpp"),
+           "-o", os.path.join(nat, "pack_check")]
     _run(cmd)
 
 

@@ -1,5 +1,6 @@
 // qs_insert.hip -- deterministic MSWS insert for gfx950: sort by bucket, then
-// every bucket segment has exactly one owner.
+// every bucket segment has exactly one owner.  One pipeline serves a plain
+// sketch and a batch of independent sketches (the multi-sketch form).
 //
 // Replaces insert_kernel / calculation_s2_insert / atomic_malloc
 // (include/sketch/device_sketch/device_sketch_v3.h:68-290).  The reference
@@ -14,16 +15,17 @@
 // size, and atomics on one 128-byte line serialise at ~11 ns.  An atomic per
 // key is therefore bounded far below the HBM roofline, and a Zipf-hot key turns
 // its three counters into a serial queue.  A stable LSD radix sort of the
-// chunk by bucket turns the table update into an address-ordered sweep with one
-// plain read-modify-write per touched bucket.
+// chunk by (sketch, bucket) turns the table update into an address-ordered
+// sweep with one plain read-modify-write per touched bucket.
 //
 // Each insert is hashed once, in the first radix pass, into a 64-bit item
-// (bucket | mask index | chunk-local stream index); later passes and kernels
+// (bucket | sketch | mask index | stream index); later passes and kernels
 // only extract bit fields.  Digit ranks inside a wavefront come from an LDS
 // match (every lane ORs its lane bit into the slot of its digit and reads the
 // group mask back), which is stable and costs three LDS operations per item.
 //
-// Per chunk:
+// Per chunk (a fixed launch sequence with no host synchronisation, so it can
+// be captured in a hipGraph; all counts stay on the device):
 //   sort      per pass: k_bs_up (tile digit counts), a scan over tiles,
 //             k_bs_down (rank, stage digit-sorted in LDS, write digit runs)
 //   k_seg     one wavefront per window of 64 sorted items: a segmented scan
@@ -36,10 +38,10 @@
 //             kernel, unlinked ones to the replay first.
 //   k_ordered one wavefront per queued segment replays it in stream order until
 //             its allocation (at most ceil(28*129/3) inserts can precede it).
-//   k_assign  allocation ids in stream order of the triggering inserts.
+//   k_assign  allocation ids per sketch, in stream order of the triggering inserts.
 //   k_tail    long segments after their allocation point: per-lane LDS counter
-//             rows over 16K-item windows, one atomic per counter per window and
-//             bucket.
+//             rows over windows of sorted items, one atomic per counter per
+//             window and bucket.
 #include "qs_internal.h"
 
 #include <cstring>
@@ -52,9 +54,9 @@
     } while (0)
 
 #define SEG_THREADS 256
-#define TAIL_WINDOW 16384u
-#define NO_BUCKET 0xffffffffu
-#define END_BUCKET 0xfffffffeu
+#define TAIL_WINDOW 4096u
+#define NO_CELL 0xffffffffffffffffull
+#define END_CELL 0xfffffffffffffffeull
 
 static int g_cu = 0;
 
@@ -74,8 +76,22 @@ static uint32_t grid_cap(uint64_t want, uint32_t per_cu)
     return want ? (uint32_t)want : 1u;
 }
 
+// sketch owning batch position pos: offs[sid] <= pos < offs[sid + 1]
+__device__ static inline uint32_t find_sid(const uint64_t *__restrict__ offs, uint32_t nsk, uint64_t pos)
+{
+    uint32_t lo = 0, hi = nsk; // invariant: offs[lo] <= pos < offs[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (offs[mid] <= pos)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // ===========================================================================
-// radix sort of the chunk by bucket
+// radix sort of the chunk by cell = (sketch, bucket)
 // ===========================================================================
 #define OS_THREADS 256
 #define OS_WAVES 4
@@ -84,40 +100,52 @@ static uint32_t grid_cap(uint64_t want, uint32_t per_cu)
 #define OS_TILE (OS_THREADS * OS_IPL)     // 4096 items per tile
 #define OS_BINS 256                       // LDS capacity for digits up to 8 bits
 
-template <int KB, bool FIRST>
-__device__ static inline uint64_t load_item(const void *__restrict__ in, uint32_t i,
-                                            const qs_hashpar &hp, const qs_pack &pk)
+// First pass: hash key number i of the chunk (batch position base + i).  A
+// single sketch (MULTI false) hashes with d0, passed by value; a batch finds
+// the owning sketch by binary search over the batch offsets.
+template <int KB, bool MULTI>
+__device__ static inline uint64_t make_item(const KeyT<KB> *__restrict__ keys, uint32_t i, uint64_t base,
+                                            const qs_sdesc &d0, const qs_sdesc *__restrict__ D,
+                                            const uint64_t *__restrict__ offs, uint32_t nsk,
+                                            const qs_pack &pk)
 {
-    if (FIRST) {
-        const KeyT<KB> key = reinterpret_cast<const KeyT<KB> *>(in)[i];
-        return (uint64_t)qs_bucket(key, hp) | ((uint64_t)qs_maskidx(key, hp) << pk.bb) |
+    const KeyT<KB> key = keys[i];
+    if (!MULTI)
+        return (uint64_t)qs_bucket_d(key, d0) | ((uint64_t)qs_maskidx_d(key, d0) << pk.mshift) |
                ((uint64_t)i << pk.ishift);
-    }
-    return reinterpret_cast<const uint64_t *>(in)[i];
+    const uint64_t pos = base + i;
+    const uint32_t sid = find_sid(offs, nsk, pos);
+    const qs_sdesc &d = D[sid];
+    const uint64_t start = offs[sid] > base ? offs[sid] : base; // this sketch's part of the chunk
+    return (uint64_t)qs_bucket_d(key, d) | ((uint64_t)sid << pk.bb) |
+           ((uint64_t)qs_maskidx_d(key, d) << pk.mshift) | ((pos - start) << pk.ishift);
 }
 
-// Per-tile digit counts, digit-major: hist[d * ntiles + tile].
-template <int KB, bool FIRST>
-__global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ in, uint32_t n,
-                                                      qs_hashpar hp, qs_pack pk, int shift,
-                                                      uint32_t dmask, uint32_t nbins,
-                                                      uint32_t ntiles, uint32_t *__restrict__ hist)
+template <int KB, bool FIRST, bool MULTI>
+__global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ in, uint32_t n, uint64_t base,
+                                                      qs_sdesc d0, const qs_sdesc *__restrict__ D,
+                                                      const uint64_t *__restrict__ offs, uint32_t nsk,
+                                                      qs_pack pk, int shift, uint32_t dmask,
+                                                      uint32_t nbins, uint32_t ntiles,
+                                                      uint32_t *__restrict__ hist)
 {
     __shared__ uint32_t s_h[OS_BINS];
     const uint32_t t = threadIdx.x;
     s_h[t] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * OS_TILE;
+    const uint32_t tb = blockIdx.x * OS_TILE;
 #pragma unroll 4
     for (int r = 0; r < OS_IPL; ++r) {
-        const uint32_t i = base + r * OS_THREADS + t;
+        const uint32_t i = tb + r * OS_THREADS + t;
         if (i < n) {
-            uint32_t b;
+            uint64_t c;
             if (FIRST)
-                b = qs_bucket(reinterpret_cast<const KeyT<KB> *>(in)[i], hp);
+                c = it_cell(make_item<KB, MULTI>(reinterpret_cast<const KeyT<KB> *>(in), i, base, d0, D, offs,
+                                                 nsk, pk),
+                            pk);
             else
-                b = it_b(reinterpret_cast<const uint64_t *>(in)[i], pk);
-            atomicAdd(&s_h[(b >> shift) & dmask], 1u);
+                c = it_cell(reinterpret_cast<const uint64_t *>(in)[i], pk);
+            atomicAdd(&s_h[(uint32_t)(c >> shift) & dmask], 1u);
         }
     }
     __syncthreads();
@@ -151,9 +179,10 @@ __device__ static inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_red
 // rounds: the LDS match gives the lanes sharing a digit, the lowest of them
 // advances the wave's running count.  The tile is then staged digit-sorted in
 // LDS and written out as contiguous digit runs.
-template <int KB, bool FIRST>
+template <int KB, bool FIRST, bool MULTI>
 __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
-    const void *__restrict__ in, uint32_t n, qs_hashpar hp, qs_pack pk, int shift, uint32_t dmask,
+    const void *__restrict__ in, uint32_t n, uint64_t base, qs_sdesc d0, const qs_sdesc *__restrict__ D,
+    const uint64_t *__restrict__ offs, uint32_t nsk, qs_pack pk, int shift, uint32_t dmask,
     uint32_t nbins, uint32_t ntiles, const uint32_t *__restrict__ hist,
     const uint32_t *__restrict__ totals, uint64_t *__restrict__ out)
 {
@@ -165,7 +194,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     __shared__ uint32_t s_red[OS_WAVES];
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
     const uint32_t tile = blockIdx.x;
-    const uint32_t base = tile * OS_TILE;
+    const uint32_t tb = tile * OS_TILE;
 #pragma unroll
     for (int w = 0; w < OS_WAVES; ++w) {
         s_wh[w][t] = 0;
@@ -186,10 +215,15 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     uint32_t dr[OS_IPL]; // digit | wave-local position << 8 | valid << 31
 #pragma unroll
     for (int r = 0; r < OS_IPL; ++r) {
-        const uint32_t i = base + wv * OS_WAVE_ITEMS + r * 64u + lane;
+        const uint32_t i = tb + wv * OS_WAVE_ITEMS + r * 64u + lane;
         const bool valid = i < n;
-        item[r] = valid ? load_item<KB, FIRST>(in, i, hp, pk) : 0ull;
-        const uint32_t d = (it_b(item[r], pk) >> shift) & dmask;
+        if (FIRST)
+            item[r] = valid ? make_item<KB, MULTI>(reinterpret_cast<const KeyT<KB> *>(in), i, base, d0, D, offs,
+                                                   nsk, pk)
+                            : 0ull;
+        else
+            item[r] = valid ? reinterpret_cast<const uint64_t *>(in)[i] : 0ull;
+        const uint32_t d = (uint32_t)(it_cell(item[r], pk) >> shift) & dmask;
         if (valid)
             atomicOr(&s_match[wv][d], (unsigned long long)(1ull << lane));
         const uint64_t m = s_match[wv][d];
@@ -218,16 +252,49 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
         }
     }
     __syncthreads();
-    const uint32_t tn = (n - base) < OS_TILE ? (n - base) : OS_TILE;
+    const uint32_t tn = (n - tb) < OS_TILE ? (n - tb) : OS_TILE;
     for (uint32_t q = t; q < tn; q += OS_THREADS) {
         const uint64_t it = s_item[q];
-        out[s_goff[(it_b(it, pk) >> shift) & dmask] + q] = it;
+        out[s_goff[(uint32_t)(it_cell(it, pk) >> shift) & dmask] + q] = it;
     }
 }
 
 // ===========================================================================
 // per-segment application
 // ===========================================================================
+
+// table pointers of one sketch
+struct seg_tables {
+    uint32_t *low;       // this sketch's low table
+    uint32_t *high;      // this sketch's high table
+    const uint32_t *masks;
+};
+
+// a single sketch's tables start at offset 0 of every pool
+template <bool MULTI>
+__device__ static inline seg_tables tables_of(uint32_t sid, const qs_sdesc *__restrict__ D,
+                                              uint32_t *low_pool, uint32_t *high_pool,
+                                              const uint32_t *mask_pool)
+{
+    seg_tables t;
+    t.low = low_pool;
+    t.high = high_pool;
+    t.masks = mask_pool;
+    if (MULTI) {
+        const qs_sdesc &d = D[sid];
+        t.low += d.low_off;
+        t.high += d.high_off;
+        t.masks += d.masks_off;
+    }
+    return t;
+}
+
+template <bool MULTI>
+__device__ static inline uint32_t mask_of(uint32_t sid, uint32_t mu, const qs_sdesc *__restrict__ D,
+                                          const uint32_t *__restrict__ mask_pool, const uint32_t *s_masks)
+{
+    return MULTI ? mask_pool[D[sid].masks_off + mu] : s_masks[mu];
+}
 
 // packed per-byte increments of one 3-bit mask into 7 words
 __device__ static inline void mask_to_words(uint32_t M, uint32_t a[7])
@@ -312,15 +379,42 @@ __device__ static inline void wave_append(bool flag, uint32_t value, uint32_t *l
         list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
 }
 
-__global__ __launch_bounds__(SEG_THREADS) void k_seg(
-    const uint64_t *__restrict__ IT, uint32_t n, uint32_t *__restrict__ low,
-    uint32_t *__restrict__ high, const uint32_t *__restrict__ masks, qs_pack pk, uint32_t n_masks,
-    uint32_t *__restrict__ longbm, uint32_t *__restrict__ ord, uint32_t *__restrict__ lng,
-    qs_dev_state *__restrict__ st)
+__device__ static inline uint64_t shfl64(uint64_t v, int src)
 {
-    __shared__ uint32_t s_masks[QS_MAX_MASKS];
-    for (uint32_t i = threadIdx.x; i < n_masks; i += SEG_THREADS)
-        s_masks[i] = masks[i];
+    uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+    uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <bool MULTI>
+__device__ static inline void set_long(uint32_t *longbm, const qs_sdesc *__restrict__ D, uint32_t sid,
+                                       uint32_t b)
+{
+    const uint64_t bit = (MULTI ? D[sid].bm_off : 0ull) + b;
+    atomicOr(longbm + (bit >> 5), 1u << (bit & 31u));
+}
+
+template <bool MULTI>
+__device__ static inline bool test_long(const uint32_t *longbm, const qs_sdesc *__restrict__ D,
+                                        uint32_t sid, uint32_t b)
+{
+    const uint64_t bit = (MULTI ? D[sid].bm_off : 0ull) + b;
+    return (longbm[bit >> 5] >> (bit & 31u)) & 1u;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(SEG_THREADS) void k_seg(
+    const uint64_t *__restrict__ IT, uint32_t n, const qs_sdesc *__restrict__ D, uint32_t n_masks0,
+    uint32_t *__restrict__ low_pool, uint32_t *__restrict__ high_pool,
+    const uint32_t *__restrict__ mask_pool, qs_pack pk, uint32_t *__restrict__ longbm,
+    uint32_t *__restrict__ ord, uint32_t *__restrict__ lng, qs_dev_state *__restrict__ st)
+{
+    // a single sketch keeps its masks in LDS; a batch reads each sketch's masks
+    // from the (cache-resident) pool
+    __shared__ uint32_t s_masks[MULTI ? 1 : QS_MAX_MASKS];
+    if (!MULTI)
+        for (uint32_t i = threadIdx.x; i < n_masks0; i += SEG_THREADS)
+            s_masks[i] = mask_pool[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t le = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
@@ -330,19 +424,23 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
         const uint32_t p = (uint32_t)w0 + lane;
         const bool valid = p < n;
         const uint64_t it = valid ? IT[p] : 0ull;
-        const uint32_t b = valid ? it_b(it, pk) : NO_BUCKET;
-        uint32_t bl = __shfl(b, (int)(lane == 0 ? 0 : lane - 1u), 64);
+        const uint64_t c = valid ? it_cell(it, pk) : NO_CELL;
+        uint64_t cl = shfl64(c, lane == 0 ? 0 : (int)lane - 1);
         if (lane == 0)
-            bl = (w0 > 0) ? it_b(IT[w0 - 1], pk) : NO_BUCKET;
+            cl = (w0 > 0) ? it_cell(IT[w0 - 1], pk) : NO_CELL;
         // a window inside a segment that started earlier has nothing to own
-        if (__ballot(valid && b != bl) == 0)
+        if (__ballot(valid && c != cl) == 0)
             continue;
-        const uint32_t M = valid ? s_masks[it_mu(it, pk)] : 0u;
-        uint32_t br = __shfl(b, (int)(lane == 63 ? 63 : lane + 1u), 64);
+        const uint32_t sid = it_sid(it, pk);
+        const uint32_t b = it_b(it, pk);
+        uint32_t M = 0;
+        if (valid)
+            M = mask_of<MULTI>(sid, it_mu(it, pk), D, mask_pool, s_masks);
+        uint64_t cr = shfl64(c, lane == 63 ? 63 : (int)lane + 1);
         if (lane == 63)
-            br = (p + 1u < n) ? it_b(IT[p + 1u], pk) : END_BUCKET;
-        const bool head = valid && b != bl;
-        const bool last = valid && b != br;
+            cr = (p + 1u < n) ? it_cell(IT[p + 1u], pk) : END_CELL;
+        const bool head = valid && c != cl;
+        const bool last = valid && c != cr;
         uint32_t a[7];
         mask_to_words(M, a);
         // segmented inclusive scan of the packed increments over runs
@@ -364,14 +462,18 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
         const uint32_t s = 63u - (uint32_t)__builtin_clzll(starts & le); // first lane of my run
         const bool owned = valid && (s > 0u || (heads & 1ull));
         bool risky = false;
-        if (owned && last)
-            risky = apply_segment(low, high, b, a);
+        if (owned && last) {
+            seg_tables tb = tables_of<MULTI>(sid, D, low_pool, high_pool, mask_pool);
+            risky = apply_segment(tb.low, tb.high, b, a);
+        }
         wave_append(risky, (uint32_t)w0 + s, ord, &st->n_ord);
         // the last run continues past the window: look one window further
         const bool ext = __shfl((int)(owned && !last), 63, 64) != 0;
         if (ext) {
-            const uint32_t b63 = __shfl(b, 63, 64);
+            const uint64_t c63 = shfl64(c, 63);
             const uint32_t s63 = __shfl(s, 63, 64);
+            const uint32_t sid63 = (uint32_t)(c63 >> pk.bb);
+            const uint32_t b63 = (uint32_t)(c63 & pk.bmask);
             uint32_t e[7];
 #pragma unroll
             for (int k = 0; k < 7; ++k)
@@ -381,36 +483,277 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
             uint32_t Mq = 0;
             if (q < n) {
                 const uint64_t iq = IT[q];
-                in = it_b(iq, pk) == b63;
-                Mq = in ? s_masks[it_mu(iq, pk)] : 0u;
+                in = it_cell(iq, pk) == c63;
+                if (in)
+                    Mq = mask_of<MULTI>(sid63, it_mu(iq, pk), D, mask_pool, s_masks);
             }
             const uint64_t im = __ballot(in);
             const uint32_t head_pos = (uint32_t)w0 + s63;
             if (im != ~0ull) {
-                uint32_t c[7];
-                mask_to_words(Mq, c);
+                uint32_t cc[7];
+                mask_to_words(Mq, cc);
 #pragma unroll
                 for (int k = 0; k < 7; ++k) {
-                    uint32_t v = c[k];
+                    uint32_t v = cc[k];
 #pragma unroll
                     for (int o = 32; o > 0; o >>= 1)
                         v += __shfl_xor(v, o, 64);
                     e[k] += v;
                 }
                 bool r2 = false;
-                if (lane == 0)
-                    r2 = apply_segment(low, high, b63, e);
+                if (lane == 0) {
+                    seg_tables tb = tables_of<MULTI>(sid63, D, low_pool, high_pool, mask_pool);
+                    r2 = apply_segment(tb.low, tb.high, b63, e);
+                }
                 wave_append(r2, head_pos, ord, &st->n_ord);
             } else {
                 // long segment: tail kernel (after the in-order replay if unlinked)
                 bool unl = false;
                 if (lane == 0) {
-                    atomicOr(longbm + (b63 >> 5), 1u << (b63 & 31u));
-                    unl = low[(size_t)b63 * 8u + 7u] <= QS_LINK_START;
+                    set_long<MULTI>(longbm, D, sid63, b63);
+                    unl = tables_of<MULTI>(sid63, D, low_pool, high_pool, mask_pool).low[(size_t)b63 * 8u + 7u] <=
+                          QS_LINK_START;
                 }
                 wave_append(lane == 0, head_pos, lng, &st->n_long);
                 wave_append(unl, head_pos, ord, &st->n_ord);
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_segw<W, IPL>: the same per-segment application with a tunable shape.  A
+// group of W lanes (a sub-wavefront) owns windows of T = W * IPL sorted items;
+// each lane reduces IPL consecutive items on its own (runs that start and end
+// inside the lane are applied by it directly), then a W-wide segmented scan
+// joins the runs that cross lanes.  A run is owned by the window holding its
+// head.  A run that crosses the window end is finished from one extension
+// window of T items; one that fills the extension, or that reaches 256 items
+// (the packed byte sums hold at most 255), is long.
+// ---------------------------------------------------------------------------
+template <int W>
+__device__ static inline uint64_t shfl64w(uint64_t v, int src)
+{
+    uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, W);
+    uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, W);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ static inline void add_words(uint32_t a[7], const uint32_t b[7])
+{
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        a[k] += b[k];
+}
+
+// one owner finishes a run: apply it, or queue it for the replay, or mark it long
+template <bool MULTI>
+__device__ static inline void finish_run(bool act, uint32_t sid, uint32_t b, uint64_t len, const uint32_t a[7],
+                                         const qs_sdesc *__restrict__ D, uint32_t *low_pool,
+                                         uint32_t *high_pool, const uint32_t *mask_pool, uint32_t *longbm,
+                                         bool &risky, bool &lng_flag, bool &unl)
+{
+    risky = lng_flag = unl = false;
+    if (!act)
+        return;
+    seg_tables tb = tables_of<MULTI>(sid, D, low_pool, high_pool, mask_pool);
+    if (len >= 256u) {
+        set_long<MULTI>(longbm, D, sid, b);
+        lng_flag = true;
+        unl = tb.low[(size_t)b * 8u + 7u] <= QS_LINK_START;
+        return;
+    }
+    risky = apply_segment(tb.low, tb.high, b, a);
+}
+
+template <bool MULTI, int W, int IPL>
+__global__ __launch_bounds__(SEG_THREADS) void k_segw(
+    const uint64_t *__restrict__ IT, uint32_t n, const qs_sdesc *__restrict__ D, uint32_t n_masks0,
+    uint32_t *__restrict__ low_pool, uint32_t *__restrict__ high_pool,
+    const uint32_t *__restrict__ mask_pool, qs_pack pk, uint32_t *__restrict__ longbm,
+    uint32_t *__restrict__ ord, uint32_t *__restrict__ lng, qs_dev_state *__restrict__ st)
+{
+    constexpr uint32_t T = (uint32_t)W * IPL;
+    __shared__ uint32_t s_masks[MULTI ? 1 : QS_MAX_MASKS];
+    if (!MULTI)
+        for (uint32_t i = threadIdx.x; i < n_masks0; i += SEG_THREADS)
+            s_masks[i] = mask_pool[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gl = lane & (uint32_t)(W - 1);
+    const uint64_t gmask = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (lane - gl));
+    const uint64_t gid = ((uint64_t)blockIdx.x * SEG_THREADS + threadIdx.x) / W;
+    const uint64_t ngroups = (uint64_t)gridDim.x * SEG_THREADS / W;
+    for (uint64_t w0 = gid * T; w0 < n; w0 += ngroups * T) {
+        const uint64_t p0 = w0 + (uint64_t)gl * IPL;
+        uint64_t c[IPL];
+        uint32_t M[IPL];
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            const uint64_t p = p0 + k;
+            const bool v = p < n;
+            const uint64_t it = v ? IT[p] : 0ull;
+            c[k] = v ? it_cell(it, pk) : NO_CELL;
+            M[k] = v ? mask_of<MULTI>(it_sid(it, pk), it_mu(it, pk), D, mask_pool, s_masks) : 0u;
+        }
+        // cell before my first item and after my last one
+        uint64_t cprev = shfl64w<W>(c[IPL - 1], (int)gl - 1);
+        if (gl == 0)
+            cprev = (w0 > 0) ? it_cell(IT[w0 - 1], pk) : NO_CELL;
+        uint64_t cnext = shfl64w<W>(c[0], (int)gl + 1);
+        if (gl == (uint32_t)(W - 1))
+            cnext = (w0 + T < n) ? it_cell(IT[w0 + T], pk) : END_CELL;
+        const bool head0 = c[0] != NO_CELL && c[0] != cprev;
+        bool anyhead = head0;
+#pragma unroll
+        for (int k = 1; k < IPL; ++k)
+            anyhead |= c[k] != NO_CELL && c[k] != c[k - 1];
+        if ((__ballot(anyhead) & gmask) == 0)
+            continue; // the window lies inside a run owned by an earlier window
+
+        // lane-local runs: first run [0, f1), interior runs, last run [rs, IPL)
+        uint32_t cur[7], af[7], tmp[7];
+        mask_to_words(M[0], cur);
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            af[k] = 0;
+        uint32_t rs = 0, f1 = IPL;
+#pragma unroll
+        for (int k = 1; k < IPL; ++k) {
+            const bool bnd = c[k] != c[k - 1];
+            // an interior run [rs, k) with rs >= 1: its head is in this lane
+            const bool inner = bnd && rs > 0 && c[k - 1] != NO_CELL;
+            bool risky = false, lf = false, un = false;
+            finish_run<MULTI>(inner, (uint32_t)(c[k - 1] >> pk.bb), (uint32_t)(c[k - 1] & pk.bmask), 1u, cur, D,
+                              low_pool, high_pool, mask_pool, longbm, risky, lf, un);
+            wave_append(risky, (uint32_t)(p0 + rs), ord, &st->n_ord);
+            if (bnd && rs == 0) {
+#pragma unroll
+                for (int j = 0; j < 7; ++j)
+                    af[j] = cur[j];
+                f1 = (uint32_t)k;
+            }
+            mask_to_words(M[k], tmp);
+            if (bnd) {
+#pragma unroll
+                for (int j = 0; j < 7; ++j)
+                    cur[j] = tmp[j];
+                rs = (uint32_t)k;
+            } else {
+                add_words(cur, tmp);
+            }
+        }
+        const bool last_head = rs > 0 || head0; // my last run starts in this lane
+        // W-wide segmented inclusive scan of the last-run sums, and an inclusive
+        // max-scan of the run heads (position + 1; 0 = before the window)
+        uint32_t x[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            x[k] = cur[k];
+        uint64_t hp = last_head ? (p0 + rs + 1u) : 0ull;
+        bool f = last_head || gl == 0;
+#pragma unroll
+        for (int o = 1; o < W; o <<= 1) {
+            const bool fu = __shfl_up((int)f, o, W) != 0;
+            const uint64_t hu = shfl64w<W>(hp, (int)gl - o);
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const uint32_t y = __shfl_up(x[k], o, W);
+                if (gl >= (uint32_t)o && !f)
+                    x[k] += y;
+            }
+            if (gl >= (uint32_t)o) {
+                hp = hu > hp ? hu : hp;
+                f = f || fu;
+            }
+        }
+        // (a) my first run ends inside this lane
+        {
+            uint32_t carry[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                carry[k] = __shfl_up(x[k], 1, W);
+            uint64_t hprev = shfl64w<W>(hp, (int)gl - 1);
+            if (gl == 0)
+                hprev = 0;
+            const uint64_t head = head0 ? p0 + 1u : hprev;
+            const bool act = f1 < (uint32_t)IPL && c[0] != NO_CELL && head != 0;
+            if (!head0) {
+#pragma unroll
+                for (int k = 0; k < 7; ++k)
+                    af[k] += carry[k];
+            }
+            bool risky, lf, un;
+            finish_run<MULTI>(act, (uint32_t)(c[0] >> pk.bb), (uint32_t)(c[0] & pk.bmask),
+                              act ? p0 + f1 - (head - 1u) : 0u, af, D, low_pool, high_pool, mask_pool, longbm,
+                              risky, lf, un);
+            const uint32_t hpos = (uint32_t)(head - 1u);
+            wave_append(risky || un, hpos, ord, &st->n_ord);
+            wave_append(lf, hpos, lng, &st->n_long);
+        }
+        // (b) my last run ends with my last item
+        const uint64_t cl = c[IPL - 1];
+        {
+            const bool act = cl != NO_CELL && cl != cnext && hp != 0;
+            bool risky, lf, un;
+            finish_run<MULTI>(act, (uint32_t)(cl >> pk.bb), (uint32_t)(cl & pk.bmask),
+                              act ? p0 + IPL - (hp - 1u) : 0u, x, D, low_pool, high_pool, mask_pool, longbm,
+                              risky, lf, un);
+            const uint32_t hpos = (uint32_t)(hp - 1u);
+            wave_append(risky || un, hpos, ord, &st->n_ord);
+            wave_append(lf, hpos, lng, &st->n_long);
+        }
+        // (c) the group's last run continues past the window: one extension window
+        const bool ext = __shfl((int)(cl != NO_CELL && cl == cnext && hp != 0), W - 1, W) != 0;
+        if (ext) {
+            const uint64_t ce = shfl64w<W>(cl, W - 1);
+            const uint64_t he = shfl64w<W>(hp, W - 1);
+            uint32_t e[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                e[k] = __shfl(x[k], W - 1, W);
+            const uint32_t sid = (uint32_t)(ce >> pk.bb), b = (uint32_t)(ce & pk.bmask);
+            uint32_t cnt = 0, sum[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                sum[k] = 0;
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                const uint64_t q = w0 + T + (uint64_t)gl * IPL + k;
+                if (q < n) {
+                    const uint64_t iq = IT[q];
+                    if (it_cell(iq, pk) == ce) {
+                        mask_to_words(mask_of<MULTI>(sid, it_mu(iq, pk), D, mask_pool, s_masks), tmp);
+                        add_words(sum, tmp);
+                        ++cnt;
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = W / 2; o > 0; o >>= 1) {
+                cnt += (uint32_t)__shfl_xor((int)cnt, o, W);
+#pragma unroll
+                for (int k = 0; k < 7; ++k)
+                    sum[k] += (uint32_t)__shfl_xor((int)sum[k], o, W);
+            }
+            add_words(e, sum);
+            const uint64_t len = w0 + T + cnt - (he - 1u);
+            const bool all_in = cnt == T;
+            bool risky = false, lf = false, un = false;
+            if (gl == 0) {
+                if (all_in) {
+                    set_long<MULTI>(longbm, D, sid, b);
+                    lf = true;
+                    un = tables_of<MULTI>(sid, D, low_pool, high_pool, mask_pool).low[(size_t)b * 8u + 7u] <=
+                         QS_LINK_START;
+                } else {
+                    finish_run<MULTI>(true, sid, b, len, e, D, low_pool, high_pool, mask_pool, longbm, risky, lf,
+                                      un);
+                }
+            }
+            const uint32_t hpos = (uint32_t)(he - 1u);
+            wave_append(risky || un, hpos, ord, &st->n_ord);
+            wave_append(lf, hpos, lng, &st->n_long);
         }
     }
 }
@@ -438,12 +781,13 @@ __device__ static inline uint32_t kth_set_bit(uint64_t m, uint32_t k)
 // the (129 - c_j)-th set bit of B_j, minimised over j.
 __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT, uint32_t n,
                                                  const uint32_t *__restrict__ ord,
-                                                 uint32_t *__restrict__ low,
-                                                 const uint32_t *__restrict__ masks, qs_pack pk,
+                                                 const qs_sdesc *__restrict__ D,
+                                                 uint32_t *__restrict__ low_pool,
+                                                 const uint32_t *__restrict__ mask_pool, qs_pack pk,
                                                  uint32_t *__restrict__ longbm,
                                                  uint32_t *__restrict__ lng,
                                                  qs_trig *__restrict__ trig, uint32_t trig_cap,
-                                                 qs_dev_state *__restrict__ st, int paper)
+                                                 qs_dev_state *__restrict__ st)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
@@ -451,21 +795,25 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
     const uint32_t n_ord = st->n_ord;
     for (uint32_t h = gw; h < n_ord; h += nw) {
         const uint32_t p0 = ord[h];
-        const uint32_t b = it_b(IT[p0], pk);
-        uint32_t *bk = low + (size_t)b * 8u;
+        const uint64_t cell = it_cell(IT[p0], pk);
+        const uint32_t b = (uint32_t)(cell & pk.bmask);
+        const uint32_t sid = (uint32_t)(cell >> pk.bb);
+        const qs_sdesc &d = D[sid];
+        uint32_t *bk = low_pool + d.low_off + (size_t)b * 8u;
+        const uint32_t *masks = mask_pool + d.masks_off;
         uint32_t word = lane < 7u ? bk[lane] : 0u;
         uint32_t c = __shfl(word, (int)(lane >> 2), 64);
         c = lane < 28u ? ((c >> ((lane & 3u) * 8u)) & 0xffu) : 0u;
         uint32_t hc = 0;
         bool fired = false, more = false;
-        uint32_t t_idx = 0, tail_start = 0;
+        uint32_t t_idx = 0, tail_start = 0, replayed = 0;
         for (uint32_t p = p0;; p += 64u) {
             const uint32_t q = p + lane;
             bool valid = false;
             uint32_t M = 0, idx = 0;
             if (q < n) {
                 const uint64_t iq = IT[q];
-                valid = it_b(iq, pk) == b;
+                valid = it_cell(iq, pk) == cell;
                 if (valid) {
                     M = masks[it_mu(iq, pk)];
                     idx = it_idx(iq, pk);
@@ -474,6 +822,7 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
             const uint64_t vm = __ballot(valid);
             if (vm == 0)
                 break;
+            replayed += (uint32_t)__popcll(vm);
             uint64_t Bj = 0;
 #pragma unroll
             for (int j = 0; j < 28; ++j) {
@@ -498,7 +847,7 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
             c += (uint32_t)__popcll(Bj & before);
             uint32_t Mt = __shfl(M, (int)pos, 64);
             t_idx = __shfl(idx, (int)pos, 64);
-            if (paper)
+            if (d.paper)
                 hc += lane < 32u ? ((Mt >> lane) & 1u) : 0u;
             else
                 hc += lane < 8u ? (uint32_t)__popc(Mt & (0x01010101u << lane)) : 0u;
@@ -514,6 +863,8 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
                       ((uint32_t)__shfl(c, (int)(4u * lane + 3u) & 63, 64) << 24);
         if (lane < 7u)
             bk[lane] = w4;
+        if (lane == 0)
+            atomicAdd(&st->replay_keys, (unsigned long long)replayed);
         if (fired) {
             uint32_t slot = 0;
             if (lane == 0) {
@@ -521,8 +872,9 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
                 // the segment continues after the batch of its allocation: make
                 // sure the tail kernel sees it
                 if (more && tail_start < n) {
-                    uint32_t old = atomicOr(longbm + (b >> 5), 1u << (b & 31u));
-                    if (!(old & (1u << (b & 31u))))
+                    const uint64_t bit = d.bm_off + b;
+                    uint32_t old = atomicOr(longbm + (bit >> 5), 1u << (bit & 31u));
+                    if (!(old & (1u << (bit & 31u))))
                         lng[atomicAdd(&st->n_long, 1u)] = p0;
                 }
             }
@@ -533,53 +885,76 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
                     tr->idx = t_idx;
                     tr->bucket = b;
                     tr->tail_start = tail_start;
-                    tr->pad = 0;
+                    tr->sid = sid;
                 }
                 if (lane < 32u)
                     tr->hc[lane] = hc;
             } else if (lane == 0) {
-                atomicOr(&st->err, 1u); // more allocations than the pool holds
+                atomicOr(&st->err, 1u); // more allocations than the pools hold
             }
         }
     }
 }
 
-__global__ void k_trig_keys(const qs_trig *__restrict__ trig, uint32_t n, uint32_t *__restrict__ tk,
+// trigger sort key: sketch << ib | stream index (fits 32 bits by construction)
+__global__ void k_trig_keys(const qs_trig *__restrict__ trig, const qs_dev_state *__restrict__ st,
+                            uint32_t trig_cap, uint32_t ib, uint32_t *__restrict__ tk,
                             uint32_t *__restrict__ tv)
 {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) {
-        tk[r] = trig[r].idx;
+    const uint32_t n = min(st->n_trig, trig_cap);
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        tk[r] = (trig[r].sid << ib) | trig[r].idx;
         tv[r] = r;
     }
 }
 
-// Ids in stream order of the triggering inserts (the sequential bump allocator
-// of v3:68-85, next starting at 1025).
+// first position of sketch sid among the sorted trigger keys
+__device__ static inline uint32_t group_start(const uint32_t *__restrict__ sk, uint32_t n, uint32_t key0)
+{
+    uint32_t lo = 0, hi = n; // first index with sk[i] >= key0
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (sk[mid] < key0)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Ids in stream order of the triggering inserts, per sketch (the sequential
+// bump allocator of v3:68-85, next starting at 1025).
 __global__ __launch_bounds__(256) void k_assign(const qs_trig *__restrict__ trig,
-                                                const uint32_t *__restrict__ order, uint32_t n_trig,
-                                                uint32_t *__restrict__ low, uint32_t *__restrict__ high,
-                                                uint32_t *__restrict__ tail_of, uint64_t n_high,
+                                                const uint32_t *__restrict__ skey,
+                                                const uint32_t *__restrict__ order, uint32_t trig_cap,
+                                                uint32_t ib, const qs_sdesc *__restrict__ D,
+                                                uint32_t *__restrict__ low_pool,
+                                                uint32_t *__restrict__ high_pool,
+                                                uint32_t *__restrict__ tail_pool,
+                                                const uint32_t *__restrict__ next,
                                                 qs_dev_state *__restrict__ st)
 {
-    const uint32_t base = st->next;
-    if ((uint64_t)(base - QS_LINK_START) + n_trig > n_high) {
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-            atomicOr(&st->err, 1u);
-        return;
-    }
+    const uint32_t n_trig = min(st->n_trig, trig_cap);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * 4u + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * 4u;
     for (uint32_t r = gw; r < n_trig; r += nw) {
         const qs_trig *tr = trig + order[r];
-        const uint32_t id = base + r;
+        const uint32_t sid = tr->sid;
+        const uint32_t start = group_start(skey, n_trig, sid << ib);
+        const qs_sdesc &d = D[sid];
+        const uint32_t id = next[sid] + (r - start);
         const uint32_t row = id - QS_LINK_START;
+        if (row >= d.n_high) {
+            if (lane == 0)
+                atomicOr(&st->err, 1u);
+            continue;
+        }
         if (lane < 32u)
-            high[(size_t)row * 32u + lane] = tr->hc[lane];
+            high_pool[d.high_off + (size_t)row * 32u + lane] = tr->hc[lane];
         if (lane == 0) {
-            low[(size_t)tr->bucket * 8u + 7u] = id;
-            tail_of[row] = tr->tail_start;
+            low_pool[d.low_off + (size_t)tr->bucket * 8u + 7u] = id;
+            tail_pool[d.tail_off + row] = tr->tail_start;
         }
     }
 }
@@ -588,19 +963,18 @@ __global__ __launch_bounds__(256) void k_assign(const qs_trig *__restrict__ trig
 // inserts after the allocation batch of a bucket linked in this chunk, are
 // commutative high adds.  A wavefront walks a window of TAIL_WINDOW sorted
 // positions; each lane counts its inserts' counters in its own LDS row while
-// the link stays the same, and the rows are summed and flushed with one atomic
-// per counter when it changes.
+// the bucket stays the same, and the rows are summed and flushed with one
+// atomic per counter when it changes.
 #define TAIL_ROW 29 // 28 counters, odd stride against bank conflicts
 
-__device__ static inline void tail_flush(uint32_t (*rows)[TAIL_ROW], uint32_t lane,
-                                         uint32_t *__restrict__ high, uint32_t cur_w)
+__device__ static inline void tail_flush(uint32_t (*rows)[TAIL_ROW], uint32_t lane, uint32_t *row)
 {
     if (lane < 28u) {
         uint32_t sum = 0;
         for (uint32_t l = 0; l < 64u; ++l)
             sum += rows[l][lane];
         if (sum)
-            atomicAdd(high + (size_t)(cur_w - QS_LINK_START) * 32u + lane, sum);
+            atomicAdd(row + lane, sum);
     }
 #pragma unroll
     for (int j = 0; j < 28; ++j)
@@ -618,18 +992,47 @@ __device__ static inline void tail_add(uint32_t (*rows)[TAIL_ROW], uint32_t lane
     }
 }
 
+// high row that receives the tail inserts of cell (sid, b), or null; tail =
+// first sorted position that goes to it
+template <bool MULTI>
+__device__ static inline uint32_t *tail_row(uint32_t sid, uint32_t b, const qs_sdesc *__restrict__ D,
+                                            const uint32_t *__restrict__ low_pool,
+                                            uint32_t *__restrict__ high_pool,
+                                            const uint32_t *__restrict__ tail_pool,
+                                            const uint32_t *__restrict__ longbm,
+                                            const uint32_t *__restrict__ first_new, uint32_t &tail)
+{
+    tail = 0xffffffffu;
+    if (!test_long<MULTI>(longbm, D, sid, b))
+        return nullptr;
+    const uint64_t low_off = MULTI ? D[sid].low_off : 0ull;
+    const uint32_t w = low_pool[low_off + (size_t)b * 8u + 7u];
+    if (w <= QS_LINK_START)
+        return nullptr;
+    const uint64_t tail_off = MULTI ? D[sid].tail_off : 0ull;
+    const uint64_t high_off = MULTI ? D[sid].high_off : 0ull;
+    tail = (w >= first_new[MULTI ? sid : 0u]) ? tail_pool[tail_off + (w - QS_LINK_START)] : 0u;
+    return high_pool + high_off + (size_t)(w - QS_LINK_START) * 32u;
+}
+
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, uint32_t n,
-                                              const uint32_t *__restrict__ low,
-                                              uint32_t *__restrict__ high,
-                                              const uint32_t *__restrict__ masks, qs_pack pk,
-                                              uint32_t n_masks, const uint32_t *__restrict__ longbm,
-                                              const uint32_t *__restrict__ tail_of,
+                                              const qs_sdesc *__restrict__ D, uint32_t n_masks0,
+                                              const uint32_t *__restrict__ low_pool,
+                                              uint32_t *__restrict__ high_pool,
+                                              const uint32_t *__restrict__ mask_pool, qs_pack pk,
+                                              const uint32_t *__restrict__ longbm,
+                                              const uint32_t *__restrict__ tail_pool,
+                                              const uint32_t *__restrict__ first_new,
                                               const qs_dev_state *__restrict__ st)
 {
-    __shared__ uint32_t s_masks[QS_MAX_MASKS];
+    if (st->n_long == 0)
+        return;
+    __shared__ uint32_t s_masks[MULTI ? 1 : QS_MAX_MASKS];
     __shared__ uint32_t s_rows[4][64][TAIL_ROW];
-    for (uint32_t i = threadIdx.x; i < n_masks; i += blockDim.x)
-        s_masks[i] = masks[i];
+    if (!MULTI)
+        for (uint32_t i = threadIdx.x; i < n_masks0; i += blockDim.x)
+            s_masks[i] = mask_pool[i];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     uint32_t(*rows)[TAIL_ROW] = s_rows[wv];
 #pragma unroll
@@ -638,85 +1041,107 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
     __syncthreads();
     const uint32_t gw = blockIdx.x * 4u + wv;
     const uint32_t nw = gridDim.x * 4u;
-    const uint32_t first_new = st->next;
     for (uint64_t w0 = (uint64_t)gw * TAIL_WINDOW; w0 < n; w0 += (uint64_t)nw * TAIL_WINDOW) {
         const uint64_t wend = (w0 + TAIL_WINDOW < n) ? w0 + TAIL_WINDOW : n;
-        uint32_t cur_w = 0;
-        // lookup cache for the bucket of the previous single-bucket step
-        uint32_t c_b = NO_BUCKET, c_w = 0, c_tail = 0;
-        bool c_long = false;
+        uint32_t *cur = nullptr;
+        // lookup cache for the cell of the previous single-cell step
+        uint64_t c_c = NO_CELL;
+        uint32_t *c_row = nullptr;
+        uint32_t c_tail = 0;
         for (uint64_t s0 = w0; s0 < wend; s0 += 64u) {
             const uint32_t p = (uint32_t)s0 + lane;
             const bool valid = p < wend;
             const uint64_t it = valid ? IT[p] : 0ull;
-            const uint32_t b = valid ? it_b(it, pk) : NO_BUCKET;
-            const uint32_t b_first = __shfl(b, 0, 64);
+            const uint64_t c = valid ? it_cell(it, pk) : NO_CELL;
+            const uint64_t c_first = shfl64(c, 0);
             const uint32_t last_lane = (wend - s0 < 64u) ? (uint32_t)(wend - s0 - 1u) : 63u;
-            const uint32_t b_last = __shfl(b, (int)last_lane, 64);
-            if (b_first == b_last) {
-                // the whole step is one bucket (the items are sorted by bucket)
-                if (b_first != c_b) {
-                    c_b = b_first;
-                    c_long = (longbm[c_b >> 5] >> (c_b & 31u)) & 1u;
-                    c_w = c_long ? low[(size_t)c_b * 8u + 7u] : 0u;
-                    c_tail = (c_w > QS_LINK_START && c_w >= first_new) ? tail_of[c_w - QS_LINK_START] : 0u;
+            const uint64_t c_last = shfl64(c, (int)last_lane);
+            if (c_first == c_last) {
+                // the whole step is one cell (the items are sorted by cell)
+                if (c_first != c_c) {
+                    c_c = c_first;
+                    c_row = tail_row<MULTI>((uint32_t)(c_c >> pk.bb), (uint32_t)(c_c & pk.bmask), D, low_pool,
+                                            high_pool, tail_pool, longbm, first_new, c_tail);
                 }
-                if (!c_long || c_w <= QS_LINK_START)
+                if (!c_row)
                     continue;
                 const bool act = valid && p >= c_tail;
                 if (!__ballot(act))
                     continue;
-                if (c_w != cur_w) {
-                    if (cur_w)
-                        tail_flush(rows, lane, high, cur_w);
-                    cur_w = c_w;
+                if (c_row != cur) {
+                    if (cur)
+                        tail_flush(rows, lane, cur);
+                    cur = c_row;
                 }
-                if (act)
-                    tail_add(rows, lane, s_masks[it_mu(it, pk)]);
+                if (act) {
+                    const uint32_t mu = it_mu(it, pk);
+                    tail_add(rows, lane, mask_of<MULTI>((uint32_t)(c >> pk.bb), mu, D, mask_pool, s_masks));
+                }
                 continue;
             }
-            bool act = false;
-            uint32_t w = 0;
-            if (valid && ((longbm[b >> 5] >> (b & 31u)) & 1u)) {
-                w = low[(size_t)b * 8u + 7u];
-                act = w > QS_LINK_START && (w < first_new || p >= tail_of[w - QS_LINK_START]);
-            }
+            uint32_t tail = 0;
+            uint32_t *row = valid ? tail_row<MULTI>((uint32_t)(c >> pk.bb), (uint32_t)(c & pk.bmask), D,
+                                                    low_pool, high_pool, tail_pool, longbm, first_new, tail)
+                                  : nullptr;
+            const bool act = row != nullptr && p >= tail;
             uint64_t am = __ballot(act);
             while (am) {
                 const uint32_t lead = (uint32_t)__builtin_ctzll(am);
-                const uint32_t wl = __shfl(w, (int)lead, 64);
-                const bool in = act && w == wl;
-                if (wl != cur_w) {
-                    if (cur_w)
-                        tail_flush(rows, lane, high, cur_w);
-                    cur_w = wl;
+                const uint64_t rl = shfl64((uint64_t)(uintptr_t)row, (int)lead);
+                const bool in = act && (uint64_t)(uintptr_t)row == rl;
+                if ((uint32_t *)(uintptr_t)rl != cur) {
+                    if (cur)
+                        tail_flush(rows, lane, cur);
+                    cur = (uint32_t *)(uintptr_t)rl;
                 }
-                if (in)
-                    tail_add(rows, lane, s_masks[it_mu(it, pk)]);
+                if (in) {
+                    const uint32_t mu = it_mu(it, pk);
+                    tail_add(rows, lane, mask_of<MULTI>((uint32_t)(c >> pk.bb), mu, D, mask_pool, s_masks));
+                }
                 am &= ~__ballot(in);
             }
         }
-        if (cur_w)
-            tail_flush(rows, lane, high, cur_w);
+        if (cur)
+            tail_flush(rows, lane, cur);
     }
 }
 
 __global__ void k_clear_long(const uint32_t *__restrict__ lng, const uint64_t *__restrict__ IT,
-                             qs_pack pk, uint32_t *__restrict__ longbm,
+                             const qs_sdesc *__restrict__ D, qs_pack pk, uint32_t *__restrict__ longbm,
                              const qs_dev_state *__restrict__ st)
 {
     const uint32_t n_long = st->n_long;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_long; i += gridDim.x * blockDim.x) {
-        const uint32_t b = it_b(IT[lng[i]], pk);
-        atomicAnd(longbm + (b >> 5), ~(1u << (b & 31u)));
+        const uint64_t it = IT[lng[i]];
+        const uint64_t bit = D[it_sid(it, pk)].bm_off + it_b(it, pk);
+        atomicAnd(longbm + (bit >> 5), ~(1u << (bit & 31u)));
     }
 }
 
-__global__ void k_finalize(qs_dev_state *st, uint32_t n_trig)
+// next[sid] += allocations of sid in this chunk: the last trigger of each
+// sketch group writes its id + 1.
+__global__ void k_next(const qs_trig *__restrict__ trig, const uint32_t *__restrict__ skey,
+                       const uint32_t *__restrict__ order, uint32_t trig_cap, uint32_t ib,
+                       uint32_t *__restrict__ next, const qs_dev_state *__restrict__ st)
+{
+    const uint32_t n_trig = min(st->n_trig, trig_cap);
+    if (st->err)
+        return;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_trig; r += gridDim.x * blockDim.x) {
+        const uint32_t sid = skey[r] >> ib;
+        if (r + 1 == n_trig || (skey[r + 1] >> ib) != sid) {
+            const uint32_t start = group_start(skey, n_trig, sid << ib);
+            next[sid] += r - start + 1;
+        }
+    }
+}
+
+__global__ void k_reset(qs_dev_state *st)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (st->err == 0)
-            st->next += n_trig;
+        st->ord_segments += st->n_ord;
+        st->long_segments += st->n_long;
+        st->chunks_ord += st->n_ord ? 1u : 0u;
         st->n_ord = 0;
         st->n_long = 0;
         st->n_trig = 0;
@@ -756,10 +1181,14 @@ void qs_free_scratch(qs_sketch *s)
 
 int qs_ensure_scratch(qs_sketch *s, uint64_t chunk)
 {
+    grid_cap(0, 1); // query the CU count now: never while a launch sequence is captured
     if (s->chunk >= chunk)
         return QS_OK;
     qs_free_scratch(s);
-    uint64_t tcap = s->p.n_high < chunk ? s->p.n_high : chunk;
+    uint64_t pool = 0;
+    for (uint32_t i = 0; i < s->nsk; ++i)
+        pool += s->ps[i].n_high;
+    uint64_t tcap = pool < chunk ? pool : chunk;
     s->trig_cap = (uint32_t)tcap;
     for (int k = 0; k < 2; ++k) {
         if (hipMalloc(&s->d_it[k], sizeof(uint64_t) * chunk) != hipSuccess ||
@@ -769,48 +1198,64 @@ int qs_ensure_scratch(qs_sketch *s, uint64_t chunk)
     }
     if (hipMalloc(&s->d_hist, sizeof(uint32_t) * qs_scratch_hist_words(chunk)) != hipSuccess ||
         hipMalloc(&s->d_ord, sizeof(uint32_t) * chunk) != hipSuccess ||
-        hipMalloc(&s->d_long, sizeof(uint32_t) * (chunk / 64 + 64)) != hipSuccess ||
+        hipMalloc(&s->d_long, sizeof(uint32_t) * (chunk / 8 + 64)) != hipSuccess ||
         hipMalloc(&s->d_trig, sizeof(qs_trig) * tcap) != hipSuccess)
         return QS_ERR_NOMEM;
     s->chunk = chunk;
     return QS_OK;
 }
 
-template <int KB>
-static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, hipStream_t stream, int &res)
+// launch context of one batch
+struct batch_ctx {
+    const uint64_t *d_offs; // device batch offsets (batches of several sketches)
+    bool timed;             // record timing events (never while capturing)
+};
+
+template <int KB, bool MULTI>
+static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t base, const batch_ctx &bc,
+                      hipStream_t stream, int &res)
 {
     const uint32_t ntiles = (cn + OS_TILE - 1) / OS_TILE;
-    const int bits = (int)s->pk.bb;
+    const int bits = (int)(s->pk.bb + s->pk.sb);
     const int passes = (bits + 7) / 8;
     const int dbits = (bits + passes - 1) / passes; // <= 8
     const uint32_t nbins = 1u << dbits, dmask = nbins - 1u;
     uint32_t *totals = s->d_hist + (size_t)OS_BINS * ntiles;
+    const qs_sdesc d0 = s->h_desc[0];
     const void *in = keys;
     int out = 0;
     for (int ps = 0; ps < passes; ++ps) {
         const int shift = dbits * ps;
         qs_evpair e0, e1, e2;
-        qs_tim_begin(s, e0, QS_TK_SORT_UP, cn, stream);
+        if (bc.timed)
+            qs_tim_begin(s, e0, QS_TK_SORT_UP, cn, stream);
         if (ps == 0)
-            hipLaunchKernelGGL((k_bs_up<KB, true>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
-                               s->hp, s->pk, shift, dmask, nbins, ntiles, s->d_hist);
+            hipLaunchKernelGGL((k_bs_up<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+                               base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
+                               s->d_hist);
         else
-            hipLaunchKernelGGL((k_bs_up<KB, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
-                               s->hp, s->pk, shift, dmask, nbins, ntiles, s->d_hist);
-        qs_tim_end(s, e0, stream);
-        qs_tim_begin(s, e1, QS_TK_SORT_SCAN, cn, stream);
+            hipLaunchKernelGGL((k_bs_up<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+                               base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
+                               s->d_hist);
+        if (bc.timed) {
+            qs_tim_end(s, e0, stream);
+            qs_tim_begin(s, e1, QS_TK_SORT_SCAN, cn, stream);
+        }
         qs_scan_digits(s->d_hist, ntiles, nbins, totals, stream);
-        qs_tim_end(s, e1, stream);
-        qs_tim_begin(s, e2, QS_TK_SORT_DOWN, cn, stream);
+        if (bc.timed) {
+            qs_tim_end(s, e1, stream);
+            qs_tim_begin(s, e2, QS_TK_SORT_DOWN, cn, stream);
+        }
         if (ps == 0)
-            hipLaunchKernelGGL((k_bs_down<KB, true>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
-                               s->hp, s->pk, shift, dmask, nbins, ntiles, s->d_hist, totals,
-                               s->d_it[out]);
+            hipLaunchKernelGGL((k_bs_down<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+                               base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
+                               s->d_hist, totals, s->d_it[out]);
         else
-            hipLaunchKernelGGL((k_bs_down<KB, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
-                               s->hp, s->pk, shift, dmask, nbins, ntiles, s->d_hist, totals,
-                               s->d_it[out]);
-        qs_tim_end(s, e2, stream);
+            hipLaunchKernelGGL((k_bs_down<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+                               base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
+                               s->d_hist, totals, s->d_it[out]);
+        if (bc.timed)
+            qs_tim_end(s, e2, stream);
         in = s->d_it[out];
         out ^= 1;
     }
@@ -819,119 +1264,187 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, hipStream
     return QS_OK;
 }
 
-static int read_state(qs_sketch *s, hipStream_t stream)
+template <bool MULTI, int W, int IPL>
+static void launch_segw(qs_sketch *s, const uint64_t *IT, uint32_t cn, hipStream_t stream)
 {
-    CK(hipMemcpyAsync(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost, stream));
-    CK(hipStreamSynchronize(stream));
-    return QS_OK;
+    const uint64_t groups = ((uint64_t)cn + W * IPL - 1) / (W * IPL);
+    hipLaunchKernelGGL((k_segw<MULTI, W, IPL>), dim3(grid_cap((groups * W + SEG_THREADS - 1) / SEG_THREADS, 8)),
+                       dim3(SEG_THREADS), 0, stream, IT, cn, s->d_desc, s->h_desc[0].n_masks, s->d_low, s->d_high,
+                       s->d_masks, s->pk, s->d_longbm, s->d_ord, s->d_long, s->d_st);
 }
 
-template <int KB>
-static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, hipStream_t stream)
+template <bool MULTI, int W>
+static void launch_segw_ipl(qs_sketch *s, const uint64_t *IT, uint32_t cn, hipStream_t stream)
+{
+    switch (s->seg_ipl) {
+    case 1: launch_segw<MULTI, W, 1>(s, IT, cn, stream); break;
+    case 2: launch_segw<MULTI, W, 2>(s, IT, cn, stream); break;
+    case 4: launch_segw<MULTI, W, 4>(s, IT, cn, stream); break;
+    default: launch_segw<MULTI, W, 8>(s, IT, cn, stream); break;
+    }
+}
+
+bool qs_seg_shape_ok(uint32_t w, uint32_t ipl)
+{
+    if (w == 0 && ipl == 0)
+        return true; // the 64-lane, one-item k_seg
+    return (w == 8 || w == 16 || w == 32 || w == 64) && (ipl == 1 || ipl == 2 || ipl == 4 || ipl == 8);
+}
+
+template <bool MULTI>
+static void launch_seg(qs_sketch *s, const uint64_t *IT, uint32_t cn, hipStream_t stream)
+{
+    switch (s->seg_w) {
+    case 8: launch_segw_ipl<MULTI, 8>(s, IT, cn, stream); break;
+    case 16: launch_segw_ipl<MULTI, 16>(s, IT, cn, stream); break;
+    case 32: launch_segw_ipl<MULTI, 32>(s, IT, cn, stream); break;
+    case 64: launch_segw_ipl<MULTI, 64>(s, IT, cn, stream); break;
+    default:
+        hipLaunchKernelGGL(k_seg<MULTI>, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(SEG_THREADS), 0,
+                           stream, IT, cn, s->d_desc, s->h_desc[0].n_masks, s->d_low, s->d_high, s->d_masks, s->pk,
+                           s->d_longbm, s->d_ord, s->d_long, s->d_st);
+        break;
+    }
+}
+
+// One chunk: a fixed launch sequence with no host synchronisation; every
+// count the later kernels need stays in device memory.
+template <int KB, bool MULTI>
+static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t base, const batch_ctx &bc,
+                        hipStream_t stream)
 {
     int res = 0;
-    int rc = sort_chunk<KB>(s, keys, cn, stream, res);
+    int rc = sort_chunk<KB, MULTI>(s, keys, cn, base, bc, stream, res);
     if (rc)
         return rc;
     const uint64_t *IT = s->d_it[res];
-    qs_evpair es;
-    qs_tim_begin(s, es, QS_TK_SEG, cn, stream);
-    hipLaunchKernelGGL(k_seg, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(SEG_THREADS), 0, stream,
-                       IT, cn, s->d_low, s->d_high, s->d_masks, s->pk, s->p.n_masks, s->d_longbm,
-                       s->d_ord, s->d_long, s->d_st);
-    qs_tim_end(s, es, stream);
-    CK(hipGetLastError());
-    rc = read_state(s, stream);
-    if (rc)
-        return rc;
-    const uint32_t n_ord = s->h_st->n_ord;
-    uint32_t n_trig = 0;
-    if (n_ord) {
-        s->chunks_fix++;
-        qs_evpair eo;
+    const uint32_t nm0 = s->h_desc[0].n_masks;
+    qs_evpair es, eo, ea, et;
+    if (bc.timed)
+        qs_tim_begin(s, es, QS_TK_SEG, cn, stream);
+    launch_seg<MULTI>(s, IT, cn, stream);
+    if (bc.timed) {
+        qs_tim_end(s, es, stream);
         qs_tim_begin(s, eo, QS_TK_ORDERED, cn, stream);
-        hipLaunchKernelGGL(k_ordered, dim3(grid_cap(((uint64_t)n_ord + 3) / 4, 8)), dim3(256), 0, stream,
-                           IT, cn, s->d_ord, s->d_low, s->d_masks, s->pk, s->d_longbm, s->d_long,
-                           s->d_trig, s->trig_cap, s->d_st, s->p.semantics == QS_PAPER ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_ordered, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(256), 0, stream, IT, cn,
+                       s->d_ord, s->d_desc, s->d_low, s->d_masks, s->pk, s->d_longbm, s->d_long, s->d_trig,
+                       s->trig_cap, s->d_st);
+    if (bc.timed) {
         qs_tim_end(s, eo, stream);
-        CK(hipGetLastError());
-        rc = read_state(s, stream);
-        if (rc)
-            return rc;
-        n_trig = s->h_st->n_trig;
-        if (n_trig > s->trig_cap)
-            n_trig = s->trig_cap;
-        if (n_trig) {
-            qs_evpair ea;
-            qs_tim_begin(s, ea, QS_TK_ASSIGN, cn, stream);
-            hipLaunchKernelGGL(k_trig_keys, dim3((n_trig + 255) / 256), dim3(256), 0, stream, s->d_trig,
-                               n_trig, s->d_tk[0], s->d_tv[0]);
-            int tbits = 32 - __builtin_clz(cn > 1 ? cn - 1 : 1);
-            int tc = qs_sort_pairs(s->d_tk, s->d_tv, s->d_hist, n_trig, tbits, stream);
-            if (tc < 0)
-                return QS_ERR_HIP;
-            hipLaunchKernelGGL(k_assign, dim3(grid_cap(((uint64_t)n_trig + 3) / 4, 8)), dim3(256), 0,
-                               stream, s->d_trig, s->d_tv[tc], n_trig, s->d_low, s->d_high,
-                               s->d_tail_of, (uint64_t)s->p.n_high, s->d_st);
-            qs_tim_end(s, ea, stream);
-        }
+        qs_tim_begin(s, ea, QS_TK_ASSIGN, cn, stream);
     }
-    if (s->h_st->n_long) {
-        qs_evpair et;
+    hipLaunchKernelGGL(k_trig_keys, dim3(grid_cap(((uint64_t)s->trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
+                       s->d_trig, s->d_st, s->trig_cap, s->pk.ib, s->d_tk[0], s->d_tv[0]);
+    const int tbits = (int)(s->pk.sb + s->pk.ib);
+    const int tc = qs_sort_pairs_dev(s->d_tk, s->d_tv, s->d_hist, &s->d_st->n_trig, s->trig_cap,
+                                     tbits > 32 ? 32 : tbits, stream);
+    if (tc < 0)
+        return QS_ERR_HIP;
+    hipLaunchKernelGGL(k_assign, dim3(grid_cap(((uint64_t)s->trig_cap + 3) / 4, 8)), dim3(256), 0, stream,
+                       s->d_trig, s->d_tk[tc], s->d_tv[tc], s->trig_cap, s->pk.ib, s->d_desc, s->d_low,
+                       s->d_high, s->d_tail_of, s->d_next, s->d_st);
+    if (bc.timed) {
+        qs_tim_end(s, ea, stream);
         qs_tim_begin(s, et, QS_TK_TAIL, cn, stream);
-        hipLaunchKernelGGL(k_tail, dim3(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8)),
-                           dim3(256), 0, stream, IT, cn, s->d_low, s->d_high, s->d_masks, s->pk,
-                           s->p.n_masks, s->d_longbm, s->d_tail_of, s->d_st);
-        hipLaunchKernelGGL(k_clear_long, dim3(grid_cap((s->h_st->n_long + 255) / 256, 4)), dim3(256), 0,
-                           stream, s->d_long, IT, s->pk, s->d_longbm, s->d_st);
+    }
+    hipLaunchKernelGGL(k_tail<MULTI>,
+                       dim3(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8)), dim3(256), 0,
+                       stream, IT, cn, s->d_desc, nm0, s->d_low, s->d_high, s->d_masks, s->pk, s->d_longbm,
+                       s->d_tail_of, s->d_next, s->d_st);
+    hipLaunchKernelGGL(k_clear_long, dim3(grid_cap(((uint64_t)cn / 64 + 255) / 256, 4)), dim3(256), 0, stream,
+                       s->d_long, IT, s->d_desc, s->pk, s->d_longbm, s->d_st);
+    hipLaunchKernelGGL(k_next, dim3(grid_cap(((uint64_t)s->trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
+                       s->d_trig, s->d_tk[tc], s->d_tv[tc], s->trig_cap, s->pk.ib, s->d_next, s->d_st);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, s->d_st);
+    if (bc.timed)
         qs_tim_end(s, et, stream);
-    }
-    s->fix_keys += n_ord;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, stream, s->d_st, n_trig);
     CK(hipGetLastError());
-    rc = read_state(s, stream);
-    if (rc)
-        return rc;
-    if (s->h_st->err) {
-        s->sticky_err = QS_ERR_POOL;
-        return QS_ERR_POOL;
-    }
     return QS_OK;
 }
 
-template <int KB>
-static int insert_impl(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream)
+// Keys per chunk.  A chunk holds at most 2^ib keys of one sketch (the stream
+// index field) and at most 2^30 keys.  A batch of several sketches whose every
+// stream fits the index field is one chunk when it fits 2^30 keys; sketch 0's
+// chunk_keys, when set, overrides that (for a set of any size).
+uint64_t qs_chunk_keys(const qs_sketch *s, const uint64_t *offs)
 {
-    const KeyT<KB> *keys = (const KeyT<KB> *)d_keys;
-    // chunk limit: the stream index field of a packed item, and 2^30
-    uint64_t cap = s->p.chunk_keys ? s->p.chunk_keys : (1ull << 28);
-    const uint32_t ibits = 64u - s->pk.ishift;
-    if (ibits < 63u && cap > (1ull << ibits))
-        cap = 1ull << ibits;
+    const uint64_t n = offs[s->nsk];
+    const uint64_t forced = s->ps[0].chunk_keys;
+    if (s->nsk > 1 && !forced && n <= (1ull << 30)) {
+        uint64_t longest = 0;
+        for (uint32_t i = 0; i < s->nsk; ++i)
+            longest = offs[i + 1] - offs[i] > longest ? offs[i + 1] - offs[i] : longest;
+        if (longest <= s->max_stream)
+            return n;
+    }
+    uint64_t cap = forced ? forced : (1ull << 28);
+    if (cap > s->max_stream)
+        cap = s->max_stream;
     if (cap > (1ull << 30))
         cap = 1ull << 30;
-    size_t done = 0;
-    while (done < n) {
-        uint64_t cn64 = n - done < cap ? n - done : cap;
-        int rc = qs_ensure_scratch(s, cn64);
-        if (rc)
-            return rc;
+    return cap;
+}
+
+template <int KB, bool MULTI>
+static int insert_impl(qs_sketch *s, const void *d_keys, const uint64_t *offs, const uint64_t *d_offs,
+                       hipStream_t stream, bool blocking, bool capture)
+{
+    const KeyT<KB> *keys = (const KeyT<KB> *)d_keys;
+    const uint64_t n = offs[s->nsk];
+    if (offs[0] != 0)
+        return QS_ERR_PARAM;
+    for (uint32_t i = 0; i < s->nsk; ++i)
+        if (offs[i + 1] < offs[i])
+            return QS_ERR_PARAM;
+    if (n == 0)
+        return QS_OK;
+    if (MULTI && !d_offs)
+        return QS_ERR_PARAM;
+    const uint64_t cap = qs_chunk_keys(s, offs);
+    const uint64_t need = n < cap ? n : cap;
+    if (s->chunk < need && s->graphs > 0)
+        return QS_ERR_STATE; // a captured graph holds the current scratch buffers
+    int rc = qs_ensure_scratch(s, need);
+    if (rc)
+        return rc;
+    batch_ctx bc;
+    bc.d_offs = d_offs;
+    bc.timed = s->timing && !capture;
+    for (uint64_t done = 0; done < n;) {
+        const uint64_t cn64 = n - done < cap ? n - done : cap;
         const uint32_t cn = (uint32_t)cn64;
-        qs_evpair ev;
-        qs_tim_begin(s, ev, QS_TIMING_KERNELS, cn, stream);
-        qs_tim_end(s, ev, stream);
-        rc = insert_chunk<KB>(s, keys + done, cn, stream);
+        if (bc.timed) {
+            qs_evpair ev;
+            qs_tim_begin(s, ev, QS_TIMING_KERNELS, cn, stream);
+            qs_tim_end(s, ev, stream);
+        }
+        rc = insert_chunk<KB, MULTI>(s, keys + done, cn, done, bc, stream);
         if (rc)
             return rc;
-        s->chunks++;
-        s->keys_inserted += cn;
+        if (!capture) {
+            s->chunks++;
+            s->keys_inserted += cn;
+        }
         done += cn;
+    }
+    if (blocking) {
+        CK(hipMemcpyAsync(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost, stream));
+        CK(hipStreamSynchronize(stream));
+        if (s->h_st->err) {
+            s->sticky_err = QS_ERR_POOL;
+            return QS_ERR_POOL;
+        }
     }
     return QS_OK;
 }
 
-int qs_insert_chunks(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream)
+int qs_insert_set(qs_sketch *s, const void *d_keys, const uint64_t *offs, const uint64_t *d_offs,
+                  hipStream_t stream, bool blocking, bool capture)
 {
-    return s->p.key_bytes == 4 ? insert_impl<4>(s, d_keys, n, stream)
-                               : insert_impl<8>(s, d_keys, n, stream);
+    if (s->nsk == 1)
+        return s->key_bytes == 4 ? insert_impl<4, false>(s, d_keys, offs, d_offs, stream, blocking, capture)
+                                 : insert_impl<8, false>(s, d_keys, offs, d_offs, stream, blocking, capture);
+    return s->key_bytes == 4 ? insert_impl<4, true>(s, d_keys, offs, d_offs, stream, blocking, capture)
+                             : insert_impl<8, true>(s, d_keys, offs, d_offs, stream, blocking, capture);
 }

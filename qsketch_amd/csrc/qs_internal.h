@@ -67,105 +67,174 @@ __host__ __device__ static inline uint32_t qs_fastmod(uint32_t n, const qs_fastd
     return n - q * f.d;
 }
 
-// Bucket index and mask index of a key (hash_mul_add, device_hash_function.h:4-18,
-// then v3:41-44).  32-bit keys hash in 32 bits (Hashed_T = uint, src/main.cu:5);
-// 64-bit keys hash in 64 bits (the functor's default Hashed_T = size_t).
-struct qs_hashpar {
-    qs_fastdiv low;
-    qs_fastdiv masks;
-    uint32_t seed;
-    uint32_t n_low;
-    uint32_t n_masks;
-};
-
 template <int KB> struct qs_key;
 template <> struct qs_key<4> { typedef uint32_t T; };
 template <> struct qs_key<8> { typedef uint64_t T; };
+template <int KB> using KeyT = typename qs_key<KB>::T;
 
-__device__ static inline void qs_hash_index(uint32_t key, const qs_hashpar &hp, uint32_t &b,
-                                            uint32_t &mu)
-{
-    uint32_t h = key * hp.seed;
-    b = qs_fastmod(h, hp.low);
-    mu = qs_fastmod(h, hp.masks);
-}
-
-__device__ static inline void qs_hash_index(uint64_t key, const qs_hashpar &hp, uint32_t &b,
-                                            uint32_t &mu)
-{
-    uint64_t h = key * (uint64_t)hp.seed;
-    b = (uint32_t)(h % (uint64_t)hp.n_low);
-    mu = (uint32_t)(h % (uint64_t)hp.n_masks);
-}
-
-// One insert after hashing: bucket | mask index << bb | chunk-local stream
-// index << (bb + bm), in 64 bits.  Built once in the first radix pass, so no
-// later pass or kernel hashes again.
-struct qs_pack {
-    uint32_t bb;     // bucket bits
-    uint32_t bm;     // mask-index bits
-    uint32_t ishift; // bb + bm
-    uint32_t mmask;  // (1 << bm) - 1
-    uint64_t bmask;  // (1 << bb) - 1
+// Device descriptor of one sketch of a set.  A plain sketch is a set of one;
+// the batched multi-sketch form holds thousands.  Every table of every sketch
+// lives in one pool per table kind, addressed through these offsets.
+struct qs_sdesc {
+    uint64_t low_off;   // words into the low pool (multiple of 8)
+    uint64_t high_off;  // words into the high pool (multiple of 32)
+    uint64_t bm_off;    // bits into the long-segment bitmap (multiple of 32)
+    uint64_t tail_off;  // entries into the tail-start pool (= high_off / 32)
+    uint32_t masks_off; // words into the mask pool
+    uint32_t n_masks;
+    uint32_t n_low;
+    uint32_t n_high;
+    uint32_t seed;      // effective hash multiplier
+    uint32_t paper;     // QS_PAPER semantics
+    qs_fastdiv fd_low;
+    qs_fastdiv fd_masks;
 };
 
+// Bucket index and mask index of a key (hash_mul_add, device_hash_function.h:4-18,
+// then v3:41-44).  32-bit keys hash in 32 bits (Hashed_T = uint, src/main.cu:5);
+// 64-bit keys hash in 64 bits (the functor's default Hashed_T = size_t).
+__device__ static inline uint32_t qs_bucket_d(uint32_t key, const qs_sdesc &d)
+{
+    return qs_fastmod(key * d.seed, d.fd_low);
+}
+__device__ static inline uint32_t qs_bucket_d(uint64_t key, const qs_sdesc &d)
+{
+    return (uint32_t)((key * (uint64_t)d.seed) % (uint64_t)d.n_low);
+}
+__device__ static inline uint32_t qs_maskidx_d(uint32_t key, const qs_sdesc &d)
+{
+    return qs_fastmod(key * d.seed, d.fd_masks);
+}
+__device__ static inline uint32_t qs_maskidx_d(uint64_t key, const qs_sdesc &d)
+{
+    return (uint32_t)((key * (uint64_t)d.seed) % (uint64_t)d.n_masks);
+}
+
+// One insert after hashing, in 64 bits, built once in the first radix pass:
+//   [ bucket : bb | sketch : sb | mask index : bm | stream index : ib ]
+// The sort key is the low bb + sb bits (bucket within sketch), so a bucket
+// segment is a run of equal "cell" values.  The stream index is the position
+// within the sketch's part of the chunk, so allocations of one sketch can be
+// ordered by (sketch, index) in 32 bits.
+struct qs_pack {
+    uint32_t bb, sb, bm, ib;
+    uint32_t mshift;   // bb + sb
+    uint32_t ishift;   // bb + sb + bm
+    uint32_t mmask;    // (1 << bm) - 1
+    uint32_t pad;
+    uint64_t bmask;    // (1 << bb) - 1
+    uint64_t cmask;    // (1 << (bb + sb)) - 1: the cell (sketch, bucket)
+};
+
+static inline uint32_t qs_bits_for(uint64_t n) // bits to hold 0 .. n-1 (at least 1)
+{
+    return n > 1 ? 64u - (uint32_t)__builtin_clzll(n - 1) : 1u;
+}
+
+// Field widths for a set of nsk sketches whose largest table has max_low
+// buckets and max_masks masks.  The stream index gets what is left of 64 bits,
+// but at most 32 - sb bits (the allocation order key (sketch, index) is 32
+// bits) and at most 30 bits (device code shifts 32-bit values by ib; a shift
+// by 32 would wrap to a shift by 0).  Returns false when fewer than 12 bits
+// remain.
+static inline bool qs_make_pack(uint64_t max_low, uint32_t nsk, uint64_t max_masks, qs_pack *pk)
+{
+    qs_pack p;
+    p.bb = qs_bits_for(max_low);
+    p.sb = nsk > 1 ? qs_bits_for(nsk) : 0u;
+    p.bm = qs_bits_for(max_masks);
+    int ib = 64 - (int)(p.bb + p.sb + p.bm);
+    ib = ib < 32 - (int)p.sb ? ib : 32 - (int)p.sb;
+    ib = ib < 30 ? ib : 30;
+    if (ib < 12)
+        return false;
+    p.ib = (uint32_t)ib;
+    p.mshift = p.bb + p.sb;
+    p.ishift = p.bb + p.sb + p.bm;
+    p.mmask = (1u << p.bm) - 1u;
+    p.pad = 0;
+    p.bmask = (1ull << p.bb) - 1ull;
+    p.cmask = (1ull << (p.bb + p.sb)) - 1ull;
+    *pk = p;
+    return true;
+}
+
+__host__ __device__ static inline uint64_t it_cell(uint64_t it, const qs_pack &pk)
+{
+    return it & pk.cmask;
+}
 __host__ __device__ static inline uint32_t it_b(uint64_t it, const qs_pack &pk)
 {
     return (uint32_t)(it & pk.bmask);
 }
+__host__ __device__ static inline uint32_t it_sid(uint64_t it, const qs_pack &pk)
+{
+    return (uint32_t)((it & pk.cmask) >> pk.bb);
+}
 __host__ __device__ static inline uint32_t it_mu(uint64_t it, const qs_pack &pk)
 {
-    return (uint32_t)(it >> pk.bb) & pk.mmask;
+    return (uint32_t)(it >> pk.mshift) & pk.mmask;
 }
 __host__ __device__ static inline uint32_t it_idx(uint64_t it, const qs_pack &pk)
 {
     return (uint32_t)(it >> pk.ishift);
 }
 
-// Small device-resident control block of one sketch.
+// Device-resident control block shared by a set.
 struct qs_dev_state {
-    uint32_t next;      // next high id (starts at 1025)
     uint32_t n_ord;     // bucket segments queued for the in-order replay
     uint32_t n_long;    // long bucket segments (handled by the windowed tail kernel)
     uint32_t n_trig;    // allocations triggered in the current chunk
-    uint32_t err;       // sticky error: 1 = pool exhausted
-    uint32_t pad[3];
+    uint32_t err;       // sticky error: 1 = a high pool is exhausted
+    // running totals since create / clear (statistics only)
+    unsigned long long replay_keys;   // inserts replayed in stream order
+    unsigned long long ord_segments;  // segments sent to the replay
+    unsigned long long long_segments; // segments handled by the tail kernel
+    unsigned long long chunks_ord;    // chunks with at least one replayed segment
 };
 
 // One allocation found by the in-order replay (filled before ids are known).
 struct qs_trig {
-    uint32_t idx;        // chunk-local stream index of the triggering insert
+    uint32_t idx;        // stream index of the triggering insert (within its sketch)
     uint32_t bucket;     // low bucket
     uint32_t tail_start; // first sorted position handled by the tail kernel
-    uint32_t pad;
+    uint32_t sid;        // sketch
     uint32_t hc[32];     // high counters accumulated up to tail_start (incl. trigger)
 };
 
-// Host-side handle.
+// Host-side handle: a set of nsk sketches (nsk == 1 for the plain API).
 struct qs_sketch {
-    qs_params p;
+    uint32_t nsk;
+    uint32_t key_bytes;
     int device;
-    uint32_t seed;
-    uint32_t *h_masks;   // host copy
-    qs_hashpar hp;
-    uint64_t chunk;      // capacity (keys) of the chunk scratch
-    int bucket_bits;     // bits of (n_low - 1)
-    qs_pack pk;          // layout of a packed (bucket, mask index, stream index) item
+    qs_params *ps;       // per sketch
+    uint32_t *seeds;     // per sketch (effective multiplier)
+    uint32_t *h_masks;   // concatenated mask tables (host copy)
+    qs_sdesc *h_desc;
+    qs_sdesc *d_desc;
+    uint64_t low_words, high_words, mask_words, bm_words, tail_words;
+    qs_pack pk;
+    uint64_t max_stream; // longest per-sketch part of one chunk (2^ib)
 
     uint32_t *d_low;
     uint32_t *d_high;
     uint32_t *d_masks;
+    uint32_t *d_next;       // per sketch: next high id (starts at 1025)
     qs_dev_state *d_st;
-    uint32_t *d_longbm;     // bitmap over buckets: segment handled by the tail kernel
-    uint64_t *d_it[2];      // packed chunk items sorted by bucket (double buffer)
-    uint32_t *d_hist;       // radix histograms (shared by both sorts)
+    uint32_t *d_longbm;     // bitmap over (sketch, bucket) cells
+    uint32_t *d_tail_of;    // per high bucket: tail start (current chunk)
+    uint64_t *d_offs;       // per sketch: start of its keys in the current batch (+ end)
+    uint64_t *h_offs;       // pinned staging for d_offs
+
+    uint64_t chunk;         // capacity (keys) of the chunk scratch
+    uint64_t *d_it[2];      // packed chunk items sorted by cell (double buffer)
+    uint32_t *d_hist;       // radix histograms (both sorts)
     uint32_t *d_ord;        // heads of segments for the in-order replay
     uint32_t *d_long;       // heads of long segments
     qs_trig *d_trig;
     uint32_t trig_cap;
     uint32_t *d_tk[2];      // trigger sort keys / values
     uint32_t *d_tv[2];
-    uint32_t *d_tail_of;    // n_high: tail start per high bucket (current chunk)
     qs_dev_state *h_st;     // pinned host mirror
 
     int timing;             // record events around kernel launches
@@ -174,9 +243,10 @@ struct qs_sketch {
 
     uint64_t keys_inserted;
     uint64_t chunks;
-    uint64_t chunks_fix;
-    uint64_t fix_keys;
     int sticky_err;
+    int graphs;             // live captured graphs (they hold the scratch pointers)
+    uint32_t seg_w;         // segment kernel shape: group width (0 = the 64-lane k_seg)
+    uint32_t seg_ipl;       //   and items per lane
 };
 
 // Stable LSD radix sort of (key, value) uint32 pairs, 8 bits per pass, over the
@@ -184,14 +254,28 @@ struct qs_sketch {
 // the buffer pair holding the sorted result.
 int qs_sort_pairs(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist, uint32_t count,
                   int key_bits, hipStream_t stream);
+// Same, with the element count read from device memory at run time (max_count
+// sizes the grid); capture-safe: no host synchronisation.
+int qs_sort_pairs_dev(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist,
+                      const uint32_t *d_count, uint32_t max_count, int key_bits, hipStream_t stream);
 size_t qs_sort_hist_words(uint32_t max_count);
 // Exclusive scan of each row hist[d][0..ntiles) for d < nbins; totals[d] = row sum.
 void qs_scan_digits(uint32_t *hist, uint32_t ntiles, uint32_t nbins, uint32_t *totals,
                     hipStream_t stream);
 
-// insert pipeline (qs_insert.hip)
-int qs_insert_chunks(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream);
+// insert pipeline (qs_insert.hip).  offs: host array of nsk + 1 batch offsets
+// (sketch i owns keys[offs[i], offs[i+1])).  blocking: synchronise at the end
+// and report a pool error; otherwise the call never waits on the device.
+// d_offs: device copy of offs (read by the kernels when nsk > 1); may be null
+// for a single sketch.  capture: the launch sequence is being recorded into a
+// graph (no timing events, no synchronisation).
+int qs_insert_set(qs_sketch *s, const void *d_keys, const uint64_t *offs, const uint64_t *d_offs,
+                  hipStream_t stream, bool blocking, bool capture);
 int qs_ensure_scratch(qs_sketch *s, uint64_t chunk);
+// segment kernel shapes the insert supports (group width x items per lane)
+bool qs_seg_shape_ok(uint32_t w, uint32_t ipl);
+// keys per chunk for a batch with host offsets offs
+uint64_t qs_chunk_keys(const qs_sketch *s, const uint64_t *offs);
 void qs_free_scratch(qs_sketch *s);
 uint64_t qs_scratch_hist_words(uint64_t chunk);
 
@@ -204,27 +288,3 @@ struct qs_evpair {
 void qs_tim_begin(qs_sketch *s, qs_evpair &e, int kind, uint64_t keys, hipStream_t st);
 void qs_tim_end(qs_sketch *s, qs_evpair &e, hipStream_t st);
 
-// ---------------------------------------------------------------------------
-// shared device helpers
-// ---------------------------------------------------------------------------
-__device__ static inline uint32_t qs_bucket(uint32_t key, const qs_hashpar &hp)
-{
-    return qs_fastmod(key * hp.seed, hp.low);
-}
-
-__device__ static inline uint32_t qs_bucket(uint64_t key, const qs_hashpar &hp)
-{
-    return (uint32_t)((key * (uint64_t)hp.seed) % (uint64_t)hp.n_low);
-}
-
-__device__ static inline uint32_t qs_maskidx(uint32_t key, const qs_hashpar &hp)
-{
-    return qs_fastmod(key * hp.seed, hp.masks);
-}
-
-__device__ static inline uint32_t qs_maskidx(uint64_t key, const qs_hashpar &hp)
-{
-    return (uint32_t)((key * (uint64_t)hp.seed) % (uint64_t)hp.n_masks);
-}
-
-template <int KB> using KeyT = typename qs_key<KB>::T;

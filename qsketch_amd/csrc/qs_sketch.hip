@@ -189,88 +189,13 @@ static void make_masks(std::default_random_engine &eng, uint32_t n, uint32_t *ou
 }
 
 // ===========================================================================
-// Device helpers
-// ===========================================================================
-
-__device__ static inline uint32_t sel_word(const uint4 &a, const uint4 &b, uint32_t k)
-{
-    uint32_t x01 = (k & 1u) ? a.y : a.x;
-    uint32_t x23 = (k & 1u) ? a.w : a.z;
-    uint32_t x45 = (k & 1u) ? b.y : b.x;
-    uint32_t x67 = (k & 1u) ? b.w : b.z;
-    uint32_t x03 = (k & 2u) ? x23 : x01;
-    uint32_t x47 = (k & 2u) ? x67 : x45;
-    return (k & 4u) ? x47 : x03;
-}
-
-__device__ static inline uint32_t byte_at(uint32_t w, uint32_t j)
-{
-    return (w >> ((j & 3u) * 8u)) & 0xffu;
-}
-
-// any byte selected by add (bytes of add are 0 or 1..3) has old value > 128
-__device__ static inline bool over_limit(uint32_t old, uint32_t add)
-{
-    bool r = false;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        r |= (((add >> (8 * k)) & 0xffu) != 0u) && (((old >> (8 * k)) & 0xffu) > QS_LOW_LIMIT);
-    return r;
-}
-
-// Split a 3-bit mask into at most three (word, packed add) pairs.
-struct low_adds {
-    uint32_t k[3];
-    uint32_t a[3];
-};
-
-__device__ static inline low_adds split_mask(uint32_t M)
-{
-    uint32_t j1 = (uint32_t)__builtin_ctz(M);
-    uint32_t M1 = M & (M - 1u);
-    uint32_t j2 = (uint32_t)__builtin_ctz(M1);
-    uint32_t M2 = M1 & (M1 - 1u);
-    uint32_t j3 = (uint32_t)__builtin_ctz(M2);
-    uint32_t k1 = j1 >> 2, k2 = j2 >> 2, k3 = j3 >> 2;
-    uint32_t a1 = 1u << ((j1 & 3u) * 8u), a2 = 1u << ((j2 & 3u) * 8u), a3 = 1u << ((j3 & 3u) * 8u);
-    low_adds r;
-    r.k[0] = k1;
-    r.a[0] = a1;
-    r.k[1] = k2;
-    r.a[1] = 0;
-    r.k[2] = k3;
-    r.a[2] = 0;
-    if (k2 == k1)
-        r.a[0] += a2;
-    else
-        r.a[1] = a2;
-    if (k3 == k1)
-        r.a[0] += a3;
-    else if (k3 == k2)
-        r.a[1] += a3;
-    else
-        r.a[2] = a3;
-    return r;
-}
-
-__device__ static inline bool bit_test(const uint32_t *bm, uint32_t b)
-{
-    return (bm[b >> 5] >> (b & 31u)) & 1u;
-}
-
-__device__ static inline uint32_t lds_mask(const uint32_t *s_masks, uint32_t mu)
-{
-    return s_masks[mu];
-}
-
-
-// ===========================================================================
 // Query: two lanes per key, each loading one 16-byte half of the 32-byte
 // bucket, so the pair issues a single coalesced request per key (a random
 // access costs one DRAM activation whatever its size, so requests per key are
 // what bounds this kernel).  Replaces search_kernel (v3:410-521):
 // est = min over the three selected byte counters
 //       + (linked ? min over the same three high counters : 0).
+// A batch of several sketches finds each key's sketch from the batch offsets.
 // ===========================================================================
 __device__ static inline uint32_t sel4(const uint4 &v, uint32_t k)
 {
@@ -279,23 +204,51 @@ __device__ static inline uint32_t sel4(const uint4 &v, uint32_t k)
     return (k & 2u) ? hi : lo;
 }
 
-template <int KB>
-__global__ __launch_bounds__(INS_THREADS) void k_query(const KeyT<KB> *__restrict__ keys, uint32_t n,
-                                                       const uint32_t *__restrict__ low,
-                                                       const uint32_t *__restrict__ high,
-                                                       const uint32_t *__restrict__ masks,
-                                                       qs_hashpar hp, uint32_t *__restrict__ out)
+__device__ static inline uint32_t query_sid(const uint64_t *__restrict__ offs, uint32_t nsk, uint64_t pos)
 {
-    __shared__ uint32_t s_masks[QS_MAX_MASKS];
-    for (uint32_t i = threadIdx.x; i < hp.n_masks; i += INS_THREADS)
-        s_masks[i] = masks[i];
+    uint32_t lo = 0, hi = nsk; // offs[lo] <= pos < offs[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (offs[mid] <= pos)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <int KB, bool MULTI>
+__global__ __launch_bounds__(INS_THREADS) void k_query(const KeyT<KB> *__restrict__ keys, uint32_t n, qs_sdesc d0,
+                                                       const qs_sdesc *__restrict__ D,
+                                                       const uint64_t *__restrict__ offs, uint32_t nsk,
+                                                       const uint32_t *__restrict__ low_pool,
+                                                       const uint32_t *__restrict__ high_pool,
+                                                       const uint32_t *__restrict__ mask_pool,
+                                                       uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t s_masks[MULTI ? 1 : QS_MAX_MASKS];
+    if (!MULTI)
+        for (uint32_t i = threadIdx.x; i < d0.n_masks; i += INS_THREADS)
+            s_masks[i] = mask_pool[i];
     __syncthreads();
     const uint32_t half = threadIdx.x & 1u;
     const uint64_t stride = (uint64_t)gridDim.x * (INS_THREADS / 2);
     for (uint64_t i = (uint64_t)blockIdx.x * (INS_THREADS / 2) + (threadIdx.x >> 1); i < n; i += stride) {
         const KeyT<KB> key = keys[i];
-        const uint32_t M = s_masks[qs_maskidx(key, hp)];
-        const uint4 v = reinterpret_cast<const uint4 *>(low + (size_t)qs_bucket(key, hp) * 8u)[half];
+        const uint32_t *low = low_pool;
+        const uint32_t *high = high_pool;
+        uint32_t M, b;
+        if (MULTI) {
+            const qs_sdesc &d = D[query_sid(offs, nsk, i)];
+            M = mask_pool[d.masks_off + qs_maskidx_d(key, d)];
+            b = qs_bucket_d(key, d);
+            low += d.low_off;
+            high += d.high_off;
+        } else {
+            M = s_masks[qs_maskidx_d(key, d0)];
+            b = qs_bucket_d(key, d0);
+        }
+        const uint4 v = reinterpret_cast<const uint4 *>(low + (size_t)b * 8u)[half];
         const uint32_t Mh = half ? (M >> 16) : (M & 0xffffu);
         uint32_t lo = 0xffffffffu;
         for (uint32_t m = Mh; m; m &= m - 1u) {
@@ -319,19 +272,10 @@ __global__ __launch_bounds__(INS_THREADS) void k_query(const KeyT<KB> *__restric
 }
 
 // ===========================================================================
-// Host: create / destroy / scratch
+// Host: create / destroy
 // ===========================================================================
 
 static int g_cu_count = 0;
-
-static int reset_state(qs_sketch *s, uint32_t next)
-{
-    qs_dev_state h;
-    memset(&h, 0, sizeof(h));
-    h.next = next;
-    CK(hipMemcpy(s->d_st, &h, sizeof(h), hipMemcpyHostToDevice));
-    return QS_OK;
-}
 
 static int validate_params(const qs_params *p)
 {
@@ -355,6 +299,33 @@ static int validate_params(const qs_params *p)
     return QS_OK;
 }
 
+// zero the device control block and restart every sketch's allocator at 1025
+static int reset_state(qs_sketch *s)
+{
+    qs_dev_state h;
+    memset(&h, 0, sizeof(h));
+    CK(hipMemcpy(s->d_st, &h, sizeof(h), hipMemcpyHostToDevice));
+    std::vector<uint32_t> nx(s->nsk, QS_LINK_START + 1);
+    CK(hipMemcpy(s->d_next, nx.data(), sizeof(uint32_t) * s->nsk, hipMemcpyHostToDevice));
+    return QS_OK;
+}
+
+// zero the control block only (counts, statistics, pool error)
+static int reset_counts(qs_sketch *s)
+{
+    qs_dev_state h;
+    memset(&h, 0, sizeof(h));
+    CK(hipMemcpy(s->d_st, &h, sizeof(h), hipMemcpyHostToDevice));
+    return QS_OK;
+}
+
+static void set_hash(qs_sdesc &d, uint32_t seed)
+{
+    d.seed = seed;
+    d.fd_low = qs_fastdiv_make(d.n_low);
+    d.fd_masks = qs_fastdiv_make(d.n_masks);
+}
+
 extern "C" void qs_destroy(qs_sketch *s)
 {
     if (!s)
@@ -369,39 +340,54 @@ extern "C" void qs_destroy(qs_sketch *s)
     hipFree(s->d_low);
     hipFree(s->d_high);
     hipFree(s->d_masks);
+    hipFree(s->d_next);
     hipFree(s->d_st);
     hipFree(s->d_longbm);
     hipFree(s->d_tail_of);
+    hipFree(s->d_offs);
+    hipFree(s->d_desc);
     if (s->h_st)
         hipHostFree(s->h_st);
+    free(s->ps);
+    free(s->seeds);
     free(s->h_masks);
+    free(s->h_desc);
     free(s);
 }
 
-static void set_hashpar(qs_sketch *s)
+static int create_set(const qs_params *ps, uint32_t nsk, qs_sketch **out)
 {
-    s->hp.low = qs_fastdiv_make((uint32_t)s->p.n_low);
-    s->hp.masks = qs_fastdiv_make(s->p.n_masks);
-    s->hp.seed = s->seed;
-    s->hp.n_low = (uint32_t)s->p.n_low;
-    s->hp.n_masks = s->p.n_masks;
-}
-
-extern "C" int qs_create(const qs_params *p, qs_sketch **out)
-{
-    if (!out)
+    if (!out || !ps || nsk < 1 || nsk > (1u << 20))
         return QS_ERR_PARAM;
     *out = nullptr;
-    int rc = validate_params(p);
-    if (rc)
-        return rc;
+    for (uint32_t i = 0; i < nsk; ++i) {
+        int rc = validate_params(ps + i);
+        if (rc)
+            return rc;
+        if (ps[i].key_bytes != ps[0].key_bytes)
+            return QS_ERR_PARAM;
+    }
+    // item packing: [bucket | sketch | mask index | stream index] in 64 bits,
+    // and (sketch, stream index) in 32 bits for the allocation order
+    uint64_t max_low = 0, max_masks = 0;
+    for (uint32_t i = 0; i < nsk; ++i) {
+        max_low = ps[i].n_low > max_low ? ps[i].n_low : max_low;
+        max_masks = ps[i].n_masks > max_masks ? ps[i].n_masks : max_masks;
+    }
+    qs_pack pk;
+    if (!qs_make_pack(max_low, nsk, max_masks, &pk))
+        return QS_ERR_PARAM; // too many sketches / too large tables for one set
+
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return QS_ERR_NODEVICE;
     qs_sketch *s = (qs_sketch *)calloc(1, sizeof(qs_sketch));
     if (!s)
         return QS_ERR_NOMEM;
-    s->p = *p;
+    s->nsk = nsk;
+    s->key_bytes = ps[0].key_bytes;
+    s->pk = pk;
+    s->max_stream = 1ull << pk.ib;
     if (hipGetDevice(&s->device) != hipSuccess) {
         free(s);
         return QS_ERR_HIP;
@@ -413,55 +399,105 @@ extern "C" int qs_create(const qs_params *p, qs_sketch **out)
         if (g_cu_count <= 0)
             g_cu_count = 256;
     }
-
-    // seeds (Device_Seed::resize, device_sketch.h:280-302) and masks (v3:558-562)
-    std::default_random_engine seed_eng, mask_eng;
-    seed_engine(seed_eng, p->rng_seed, 0);
-    seed_engine(mask_eng, p->rng_seed, 1);
-    uint32_t seed = 0;
-    {
-        std::uniform_int_distribution<unsigned int> dis(1, std::numeric_limits<unsigned int>::max());
-        for (uint32_t i = 0; i < p->seed_sz; ++i)
-            seed += dis(seed_eng);
-    }
-    s->seed = p->hash_seed ? p->hash_seed : seed;
-    s->h_masks = (uint32_t *)malloc(sizeof(uint32_t) * p->n_masks);
-    if (!s->h_masks) {
-        free(s);
+    s->ps = (qs_params *)malloc(sizeof(qs_params) * nsk);
+    s->seeds = (uint32_t *)malloc(sizeof(uint32_t) * nsk);
+    s->h_desc = (qs_sdesc *)calloc(nsk, sizeof(qs_sdesc));
+    uint64_t mask_words = 0;
+    for (uint32_t i = 0; i < nsk; ++i)
+        mask_words += ps[i].n_masks;
+    s->h_masks = (uint32_t *)malloc(sizeof(uint32_t) * mask_words);
+    if (!s->ps || !s->seeds || !s->h_desc || !s->h_masks) {
+        qs_destroy(s);
         return QS_ERR_NOMEM;
     }
-    make_masks(mask_eng, p->n_masks, s->h_masks);
-    set_hashpar(s);
+    memcpy(s->ps, ps, sizeof(qs_params) * nsk);
 
-    size_t low_bytes = (size_t)p->n_low * 32u, high_bytes = (size_t)p->n_high * 128u;
-    size_t viol_words = (size_t)((p->n_low + 31) / 32);
-    s->bucket_bits = p->n_low > 1 ? 64 - __builtin_clzll(p->n_low - 1) : 0;
-    s->pk.bb = s->bucket_bits > 0 ? (uint32_t)s->bucket_bits : 1u;
-    s->pk.bm = p->n_masks > 1 ? 32u - (uint32_t)__builtin_clz(p->n_masks - 1u) : 1u;
-    s->pk.ishift = s->pk.bb + s->pk.bm;
-    s->pk.mmask = (1u << s->pk.bm) - 1u;
-    s->pk.bmask = (1ull << s->pk.bb) - 1ull;
-    bool ok = hipMalloc(&s->d_low, low_bytes) == hipSuccess &&
-              hipMalloc(&s->d_high, high_bytes) == hipSuccess &&
-              hipMalloc(&s->d_masks, sizeof(uint32_t) * p->n_masks) == hipSuccess &&
+    uint64_t low_w = 0, high_w = 0, bm_bits = 0, tail_e = 0, mask_w = 0;
+    for (uint32_t i = 0; i < nsk; ++i) {
+        const qs_params &p = ps[i];
+        // seeds (Device_Seed::resize, device_sketch.h:280-302) and masks (v3:558-562)
+        std::default_random_engine seed_eng, mask_eng;
+        seed_engine(seed_eng, p.rng_seed, 0);
+        seed_engine(mask_eng, p.rng_seed, 1);
+        uint32_t seed = 0;
+        std::uniform_int_distribution<unsigned int> dis(1, std::numeric_limits<unsigned int>::max());
+        for (uint32_t k = 0; k < p.seed_sz; ++k)
+            seed += dis(seed_eng);
+        s->seeds[i] = p.hash_seed ? p.hash_seed : seed;
+        make_masks(mask_eng, p.n_masks, s->h_masks + mask_w);
+        qs_sdesc &d = s->h_desc[i];
+        d.low_off = low_w;
+        d.high_off = high_w;
+        d.bm_off = bm_bits;
+        d.tail_off = tail_e;
+        d.masks_off = (uint32_t)mask_w;
+        d.n_masks = p.n_masks;
+        d.n_low = (uint32_t)p.n_low;
+        d.n_high = (uint32_t)p.n_high;
+        d.paper = p.semantics == QS_PAPER;
+        set_hash(d, s->seeds[i]);
+        low_w += p.n_low * 8u;
+        high_w += p.n_high * 32u;
+        bm_bits += (p.n_low + 31) & ~31ull;
+        tail_e += p.n_high;
+        mask_w += p.n_masks;
+    }
+    s->low_words = low_w;
+    s->high_words = high_w;
+    s->bm_words = bm_bits / 32;
+    s->tail_words = tail_e;
+    s->mask_words = mask_w;
+    bool ok = hipMalloc(&s->d_low, low_w * 4) == hipSuccess && hipMalloc(&s->d_high, high_w * 4) == hipSuccess &&
+              hipMalloc(&s->d_masks, mask_w * 4) == hipSuccess &&
+              hipMalloc(&s->d_next, sizeof(uint32_t) * nsk) == hipSuccess &&
               hipMalloc(&s->d_st, sizeof(qs_dev_state)) == hipSuccess &&
-              hipMalloc(&s->d_longbm, sizeof(uint32_t) * viol_words) == hipSuccess &&
-              hipMalloc(&s->d_tail_of, sizeof(uint32_t) * p->n_high) == hipSuccess &&
+              hipMalloc(&s->d_longbm, s->bm_words * 4) == hipSuccess &&
+              hipMalloc(&s->d_tail_of, tail_e * 4) == hipSuccess &&
+              hipMalloc(&s->d_offs, sizeof(uint64_t) * (nsk + 1)) == hipSuccess &&
+              hipMalloc(&s->d_desc, sizeof(qs_sdesc) * nsk) == hipSuccess &&
               hipHostMalloc(&s->h_st, sizeof(qs_dev_state), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
         qs_destroy(s);
         return QS_ERR_NOMEM;
     }
-    if (hipMemset(s->d_low, 0, low_bytes) != hipSuccess ||
-        hipMemset(s->d_high, 0, high_bytes) != hipSuccess ||
-        hipMemset(s->d_longbm, 0, sizeof(uint32_t) * viol_words) != hipSuccess ||
-        hipMemcpy(s->d_masks, s->h_masks, sizeof(uint32_t) * p->n_masks, hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        reset_state(s, QS_LINK_START + 1) != QS_OK || hipDeviceSynchronize() != hipSuccess) {
+    if (hipMemset(s->d_low, 0, low_w * 4) != hipSuccess || hipMemset(s->d_high, 0, high_w * 4) != hipSuccess ||
+        hipMemset(s->d_longbm, 0, s->bm_words * 4) != hipSuccess ||
+        hipMemcpy(s->d_masks, s->h_masks, mask_w * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_desc, s->h_desc, sizeof(qs_sdesc) * nsk, hipMemcpyHostToDevice) != hipSuccess ||
+        reset_state(s) != QS_OK || hipDeviceSynchronize() != hipSuccess) {
         qs_destroy(s);
         return QS_ERR_HIP;
     }
     *out = s;
+    return QS_OK;
+}
+
+extern "C" int qs_create(const qs_params *p, qs_sketch **out)
+{
+    if (!out)
+        return QS_ERR_PARAM;
+    *out = nullptr;
+    int rc = validate_params(p);
+    if (rc)
+        return rc;
+    return create_set(p, 1, out);
+}
+
+extern "C" int qs_multi_create(const qs_params *ps, uint32_t n_sketches, qs_sketch **out)
+{
+    return create_set(ps, n_sketches, out);
+}
+
+extern "C" uint32_t qs_multi_count(const qs_sketch *s)
+{
+    return s ? s->nsk : 0u;
+}
+
+extern "C" int qs_multi_params(const qs_sketch *s, uint32_t sketch, qs_params *p)
+{
+    if (!s || !p || sketch >= s->nsk)
+        return QS_ERR_PARAM;
+    *p = s->ps[sketch];
     return QS_OK;
 }
 
@@ -470,14 +506,15 @@ extern "C" int qs_clear(qs_sketch *s)
     if (!s)
         return QS_ERR_PARAM;
     CK(hipDeviceSynchronize());
-    CK(hipMemset(s->d_low, 0, (size_t)s->p.n_low * 32u));
-    CK(hipMemset(s->d_high, 0, (size_t)s->p.n_high * 128u));
-    CK(hipMemset(s->d_longbm, 0, sizeof(uint32_t) * (size_t)((s->p.n_low + 31) / 32)));
-    int rc = reset_state(s, QS_LINK_START + 1);
+    CK(hipMemset(s->d_low, 0, s->low_words * 4));
+    CK(hipMemset(s->d_high, 0, s->high_words * 4));
+    CK(hipMemset(s->d_longbm, 0, s->bm_words * 4));
+    int rc = reset_state(s);
     if (rc)
         return rc;
     CK(hipDeviceSynchronize());
     s->keys_inserted = 0;
+    s->chunks = 0;
     s->sticky_err = 0;
     return QS_OK;
 }
@@ -491,59 +528,111 @@ static uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t per_cu)
     return blocks ? (uint32_t)blocks : 1u;
 }
 
-static int insert_dispatch(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream)
+// ===========================================================================
+// Host: insert / query entry points
+// ===========================================================================
+static int insert_single(qs_sketch *s, const void *d_keys, size_t n, hipStream_t stream, bool blocking)
 {
     if (!s || (!d_keys && n))
+        return QS_ERR_PARAM;
+    if (s->nsk != 1)
         return QS_ERR_PARAM;
     if (s->sticky_err)
         return s->sticky_err;
     if (n == 0)
         return QS_OK;
-    return qs_insert_chunks(s, d_keys, n, stream);
+    const uint64_t offs[2] = {0, (uint64_t)n};
+    return qs_insert_set(s, d_keys, offs, nullptr, stream, blocking, false);
 }
 
 extern "C" int qs_insert(qs_sketch *s, const void *d_keys, size_t n)
 {
-    int rc = insert_dispatch(s, d_keys, n, 0);
-    if (rc)
-        return rc;
-    CK(hipStreamSynchronize(0));
-    return QS_OK;
+    return insert_single(s, d_keys, n, 0, true);
 }
 
 extern "C" int qs_insert_async(qs_sketch *s, const void *d_keys, size_t n, void *stream)
 {
-    return insert_dispatch(s, d_keys, n, (hipStream_t)stream);
+    return insert_single(s, d_keys, n, (hipStream_t)stream, false);
 }
 
-static int query_dispatch(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_counts,
-                          hipStream_t stream)
+static int check_offs(const qs_sketch *s, const uint64_t *h_offs)
 {
-    if (!s || ((!d_keys || !d_counts) && n))
+    if (!h_offs || h_offs[0] != 0)
         return QS_ERR_PARAM;
+    for (uint32_t i = 0; i < s->nsk; ++i)
+        if (h_offs[i + 1] < h_offs[i])
+            return QS_ERR_PARAM;
+    return QS_OK;
+}
+
+extern "C" int qs_multi_insert(qs_sketch *s, const void *d_keys, const uint64_t *h_offs)
+{
+    if (!s || check_offs(s, h_offs) || (!d_keys && h_offs[s->nsk]))
+        return QS_ERR_PARAM;
+    if (s->sticky_err)
+        return s->sticky_err;
+    if (h_offs[s->nsk] == 0)
+        return QS_OK;
+    CK(hipDeviceSynchronize());
+    CK(hipMemcpy(s->d_offs, h_offs, sizeof(uint64_t) * (s->nsk + 1), hipMemcpyHostToDevice));
+    return qs_insert_set(s, d_keys, h_offs, s->d_offs, 0, true, false);
+}
+
+extern "C" int qs_multi_insert_async(qs_sketch *s, const void *d_keys, const uint64_t *h_offs,
+                                     const uint64_t *d_offs, void *stream)
+{
+    if (!s || check_offs(s, h_offs) || (!d_keys && h_offs[s->nsk]) || (!d_offs && s->nsk > 1))
+        return QS_ERR_PARAM;
+    if (s->sticky_err)
+        return s->sticky_err;
+    if (h_offs[s->nsk] == 0)
+        return QS_OK;
+    return qs_insert_set(s, d_keys, h_offs, d_offs, (hipStream_t)stream, false, false);
+}
+
+static int query_launch(qs_sketch *s, const void *d_keys, uint64_t n, const uint64_t *d_offs, uint32_t *d_counts,
+                        hipStream_t stream)
+{
     if (n == 0)
         return QS_OK;
     if (n > 0xffffffffull)
         return QS_ERR_PARAM;
-    uint32_t nn = (uint32_t)n;
+    const uint32_t nn = (uint32_t)n;
+    const qs_sdesc d0 = s->h_desc[0];
+    const bool multi = s->nsk > 1;
+    const dim3 grid(grid_for((uint64_t)nn * 2, INS_THREADS, 8)), block(INS_THREADS);
     qs_evpair ev;
     qs_tim_begin(s, ev, QS_TK_QUERY, nn, stream);
-    if (s->p.key_bytes == 4)
-        hipLaunchKernelGGL(k_query<4>, dim3(grid_for((uint64_t)nn * 2, INS_THREADS, 8)), dim3(INS_THREADS), 0, stream,
-                           (const uint32_t *)d_keys, nn, s->d_low, s->d_high, s->d_masks, s->hp,
-                           d_counts);
-    else
-        hipLaunchKernelGGL(k_query<8>, dim3(grid_for((uint64_t)nn * 2, INS_THREADS, 8)), dim3(INS_THREADS), 0, stream,
-                           (const uint64_t *)d_keys, nn, s->d_low, s->d_high, s->d_masks, s->hp,
-                           d_counts);
+    if (s->key_bytes == 4) {
+        if (multi)
+            hipLaunchKernelGGL((k_query<4, true>), grid, block, 0, stream, (const uint32_t *)d_keys, nn, d0,
+                               s->d_desc, d_offs, s->nsk, s->d_low, s->d_high, s->d_masks, d_counts);
+        else
+            hipLaunchKernelGGL((k_query<4, false>), grid, block, 0, stream, (const uint32_t *)d_keys, nn, d0,
+                               s->d_desc, d_offs, s->nsk, s->d_low, s->d_high, s->d_masks, d_counts);
+    } else {
+        if (multi)
+            hipLaunchKernelGGL((k_query<8, true>), grid, block, 0, stream, (const uint64_t *)d_keys, nn, d0,
+                               s->d_desc, d_offs, s->nsk, s->d_low, s->d_high, s->d_masks, d_counts);
+        else
+            hipLaunchKernelGGL((k_query<8, false>), grid, block, 0, stream, (const uint64_t *)d_keys, nn, d0,
+                               s->d_desc, d_offs, s->nsk, s->d_low, s->d_high, s->d_masks, d_counts);
+    }
     qs_tim_end(s, ev, stream);
     CK(hipGetLastError());
     return QS_OK;
 }
 
+static int query_single(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_counts, hipStream_t stream)
+{
+    if (!s || ((!d_keys || !d_counts) && n) || s->nsk != 1)
+        return QS_ERR_PARAM;
+    return query_launch(s, d_keys, n, nullptr, d_counts, stream);
+}
+
 extern "C" int qs_query(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_counts)
 {
-    int rc = query_dispatch(s, d_keys, n, d_counts, 0);
+    int rc = query_single(s, d_keys, n, d_counts, 0);
     if (rc)
         return rc;
     CK(hipStreamSynchronize(0));
@@ -553,7 +642,31 @@ extern "C" int qs_query(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_
 extern "C" int qs_query_async(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_counts,
                               void *stream)
 {
-    return query_dispatch(s, d_keys, n, d_counts, (hipStream_t)stream);
+    return query_single(s, d_keys, n, d_counts, (hipStream_t)stream);
+}
+
+extern "C" int qs_multi_query(qs_sketch *s, const void *d_keys, const uint64_t *h_offs, uint32_t *d_counts)
+{
+    if (!s || check_offs(s, h_offs) || ((!d_keys || !d_counts) && h_offs[s->nsk]))
+        return QS_ERR_PARAM;
+    const uint64_t n = h_offs[s->nsk];
+    if (n == 0)
+        return QS_OK;
+    CK(hipDeviceSynchronize());
+    CK(hipMemcpy(s->d_offs, h_offs, sizeof(uint64_t) * (s->nsk + 1), hipMemcpyHostToDevice));
+    int rc = query_launch(s, d_keys, n, s->d_offs, d_counts, 0);
+    if (rc)
+        return rc;
+    CK(hipStreamSynchronize(0));
+    return QS_OK;
+}
+
+extern "C" int qs_multi_query_async(qs_sketch *s, const void *d_keys, const uint64_t *d_offs, uint64_t n,
+                                    uint32_t *d_counts, void *stream)
+{
+    if (!s || ((!d_keys || !d_counts || (!d_offs && s->nsk > 1)) && n))
+        return QS_ERR_PARAM;
+    return query_launch(s, d_keys, n, d_offs, d_counts, (hipStream_t)stream);
 }
 
 extern "C" int qs_sync(qs_sketch *s)
@@ -561,65 +674,307 @@ extern "C" int qs_sync(qs_sketch *s)
     if (!s)
         return QS_ERR_PARAM;
     CK(hipDeviceSynchronize());
+    CK(hipMemcpy(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost));
+    if (s->h_st->err)
+        s->sticky_err = QS_ERR_POOL;
     return s->sticky_err;
+}
+
+// ===========================================================================
+// Captured insert (hipGraph): the whole launch sequence of one batch layout,
+// replayed with one call.  The graph reads the keys and the device offsets
+// at replay time, so the caller may refill them between replays.
+// ===========================================================================
+struct qs_graph {
+    qs_sketch *s;
+    hipGraph_t g;
+    hipGraphExec_t x;
+    uint64_t keys;
+    uint64_t chunks;
+};
+
+extern "C" int qs_graph_capture_insert(qs_sketch *s, const void *d_keys, const uint64_t *h_offs,
+                                       const uint64_t *d_offs, qs_graph **out)
+{
+    if (!s || !out || check_offs(s, h_offs) || !d_keys || h_offs[s->nsk] == 0 || (!d_offs && s->nsk > 1))
+        return QS_ERR_PARAM;
+    *out = nullptr;
+    if (s->sticky_err)
+        return s->sticky_err;
+    // size the scratch first: allocation cannot be captured
+    const uint64_t n = h_offs[s->nsk];
+    const uint64_t cap = qs_chunk_keys(s, h_offs);
+    int rc = qs_ensure_scratch(s, n < cap ? n : cap);
+    if (rc)
+        return rc;
+    CK(hipDeviceSynchronize());
+    hipStream_t cs;
+    CK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    qs_graph *g = (qs_graph *)calloc(1, sizeof(qs_graph));
+    if (!g) {
+        hipStreamDestroy(cs);
+        return QS_ERR_NOMEM;
+    }
+    g->s = s;
+    g->keys = n;
+    g->chunks = (n + cap - 1) / cap;
+    bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        rc = qs_insert_set(s, d_keys, h_offs, d_offs, cs, false, true);
+        ok = hipStreamEndCapture(cs, &g->g) == hipSuccess && rc == QS_OK;
+    }
+    if (ok)
+        ok = hipGraphInstantiate(&g->x, g->g, nullptr, nullptr, 0) == hipSuccess;
+    hipStreamDestroy(cs);
+    if (!ok) {
+        if (g->g)
+            hipGraphDestroy(g->g);
+        free(g);
+        return rc ? rc : QS_ERR_HIP;
+    }
+    s->graphs++;
+    *out = g;
+    return QS_OK;
+}
+
+extern "C" int qs_graph_launch(qs_graph *g, void *stream)
+{
+    if (!g)
+        return QS_ERR_PARAM;
+    if (g->s->sticky_err)
+        return g->s->sticky_err;
+    CK(hipGraphLaunch(g->x, (hipStream_t)stream));
+    g->s->keys_inserted += g->keys;
+    g->s->chunks += g->chunks;
+    return QS_OK;
+}
+
+extern "C" void qs_graph_destroy(qs_graph *g)
+{
+    if (!g)
+        return;
+    hipDeviceSynchronize();
+    hipGraphExecDestroy(g->x);
+    hipGraphDestroy(g->g);
+    g->s->graphs--;
+    free(g);
+}
+
+// ===========================================================================
+// Insert shape and its autotuner.  The segment kernel's group width and items
+// per lane change only the schedule, never the result: every shape leaves the
+// same state.  The tuner times each shape on a twin set (same parameters, so
+// the caller's set is untouched), checks that all twins end in the same state
+// (a checksum of both pools), and keeps the shape with the fastest segment pass.
+// ===========================================================================
+__global__ void k_checksum(const uint32_t *__restrict__ a, uint64_t n, uint64_t salt,
+                           unsigned long long *__restrict__ out)
+{
+    uint64_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        acc += (uint64_t)a[i] * (2u * (i + salt) + 1u);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        acc += (uint64_t)__shfl_xor((long long)acc, o, 64);
+    if ((threadIdx.x & 63u) == 0)
+        atomicAdd(out, (unsigned long long)acc);
+}
+
+static int set_checksum(qs_sketch *s, uint64_t *sum)
+{
+    unsigned long long *d = nullptr;
+    CK(hipMalloc(&d, sizeof(unsigned long long)));
+    hipMemset(d, 0, sizeof(unsigned long long));
+    hipLaunchKernelGGL(k_checksum, dim3(grid_for(s->low_words, 256, 4)), dim3(256), 0, 0, s->d_low, s->low_words,
+                       0ull, d);
+    hipLaunchKernelGGL(k_checksum, dim3(grid_for(s->high_words, 256, 4)), dim3(256), 0, 0, s->d_high,
+                       s->high_words, s->low_words, d);
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess)
+        return QS_ERR_HIP;
+    *sum = h;
+    return QS_OK;
+}
+
+extern "C" int qs_set_insert_shape(qs_sketch *s, uint32_t width, uint32_t items_per_lane)
+{
+    if (!s || !qs_seg_shape_ok(width, items_per_lane))
+        return QS_ERR_PARAM;
+    if (s->graphs > 0)
+        return QS_ERR_STATE; // captured graphs hold the current shape
+    s->seg_w = width;
+    s->seg_ipl = items_per_lane;
+    return QS_OK;
+}
+
+extern "C" int qs_get_insert_shape(const qs_sketch *s, uint32_t *width, uint32_t *items_per_lane)
+{
+    if (!s || !width || !items_per_lane)
+        return QS_ERR_PARAM;
+    *width = s->seg_w;
+    *items_per_lane = s->seg_ipl;
+    return QS_OK;
+}
+
+extern "C" int qs_autotune_insert(qs_sketch *s, const void *d_keys, const uint64_t *h_offs, const uint64_t *d_offs,
+                                  int reps, qs_tune_result *out)
+{
+    if (!s || !out || check_offs(s, h_offs) || !d_keys || h_offs[s->nsk] == 0 || (!d_offs && s->nsk > 1) ||
+        reps < 1)
+        return QS_ERR_PARAM;
+    if (s->graphs > 0)
+        return QS_ERR_STATE;
+    memset(out, 0, sizeof(*out));
+    qs_sketch *t = nullptr;
+    int rc = create_set(s->ps, s->nsk, &t);
+    if (rc)
+        return rc;
+    static const uint32_t widths[4] = {8, 16, 32, 64}, ipls[4] = {1, 2, 4, 8};
+    double best = 1e300;
+    uint64_t ref_sum = 0;
+    out->consistent = 1;
+    for (int wi = 0; wi < 4 && rc == QS_OK; ++wi)
+        for (int ii = 0; ii < 4 && rc == QS_OK; ++ii) {
+            qs_tune_entry &e = out->e[out->n];
+            e.width = widths[wi];
+            e.items_per_lane = ipls[ii];
+            t->seg_w = e.width;
+            t->seg_ipl = e.items_per_lane;
+            double seg = 0, ins = 0;
+            for (int r = 0; r <= reps && rc == QS_OK; ++r) { // the first run is a warm-up
+                rc = qs_clear(t);
+                if (rc)
+                    break;
+                rc = qs_set_timing(t, r > 0);
+                if (rc)
+                    break;
+                rc = qs_insert_set(t, d_keys, h_offs, d_offs, 0, true, false);
+                if (rc)
+                    break;
+                if (r > 0) {
+                    qs_timing tm;
+                    rc = qs_get_timing(t, &tm);
+                    seg += tm.kernel_ms[QS_TK_SEG];
+                    ins += tm.insert_kernel_ms;
+                }
+            }
+            if (rc)
+                break;
+            qs_set_timing(t, 0);
+            e.seg_ms = seg / reps;
+            e.insert_ms = ins / reps;
+            rc = set_checksum(t, &e.checksum);
+            if (rc)
+                break;
+            if (out->n == 0)
+                ref_sum = e.checksum;
+            else if (e.checksum != ref_sum)
+                out->consistent = 0;
+            if (e.seg_ms < best) {
+                best = e.seg_ms;
+                out->best = out->n;
+            }
+            out->n++;
+        }
+    qs_destroy(t);
+    if (rc)
+        return rc;
+    if (!out->consistent)
+        return QS_ERR_STATE; // shapes disagree: keep the current shape
+    s->seg_w = out->e[out->best].width;
+    s->seg_ipl = out->e[out->best].items_per_lane;
+    return QS_OK;
 }
 
 // ===========================================================================
 // State export / import (reference layout) and statistics
 // ===========================================================================
-extern "C" int qs_export_state(qs_sketch *s, uint32_t *low, uint32_t *high, uint32_t *masks,
-                               uint32_t *seed, uint32_t *next)
+static int export_at(qs_sketch *s, uint32_t sid, uint32_t *low, uint32_t *high, uint32_t *masks, uint32_t *seed,
+                     uint32_t *next)
 {
-    if (!s)
+    if (!s || sid >= s->nsk)
         return QS_ERR_PARAM;
+    const qs_sdesc &d = s->h_desc[sid];
     CK(hipDeviceSynchronize());
     if (low)
-        CK(hipMemcpy(low, s->d_low, (size_t)s->p.n_low * 32u, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(low, s->d_low + d.low_off, (size_t)d.n_low * 32u, hipMemcpyDeviceToHost));
     if (high)
-        CK(hipMemcpy(high, s->d_high, (size_t)s->p.n_high * 128u, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(high, s->d_high + d.high_off, (size_t)d.n_high * 128u, hipMemcpyDeviceToHost));
     if (masks)
-        memcpy(masks, s->h_masks, sizeof(uint32_t) * s->p.n_masks);
+        memcpy(masks, s->h_masks + d.masks_off, sizeof(uint32_t) * d.n_masks);
     if (seed)
-        *seed = s->seed;
-    if (next) {
-        qs_dev_state h;
-        CK(hipMemcpy(&h, s->d_st, sizeof(h), hipMemcpyDeviceToHost));
-        *next = h.next;
-    }
+        *seed = d.seed;
+    if (next)
+        CK(hipMemcpy(next, s->d_next + sid, sizeof(uint32_t), hipMemcpyDeviceToHost));
     return QS_OK;
 }
 
-extern "C" int qs_import_state(qs_sketch *s, const uint32_t *low, const uint32_t *high,
-                               const uint32_t *masks, uint32_t seed, uint32_t next)
+static int import_at(qs_sketch *s, uint32_t sid, const uint32_t *low, const uint32_t *high, const uint32_t *masks,
+                     uint32_t seed, uint32_t next)
 {
-    if (!s || !low || !high || !masks)
+    if (!s || sid >= s->nsk || !low || !high || !masks)
         return QS_ERR_PARAM;
-    const uint64_t n_high = s->p.n_high;
+    if (s->graphs > 0)
+        return QS_ERR_STATE; // captured graphs hold the hash parameters
+    qs_sdesc &d = s->h_desc[sid];
+    const uint64_t n_high = d.n_high;
     if (next < QS_LINK_START + 1 || (uint64_t)(next - QS_LINK_START) > n_high)
         return QS_ERR_STATE;
-    for (uint32_t i = 0; i < s->p.n_masks; ++i)
+    for (uint32_t i = 0; i < d.n_masks; ++i)
         if (__builtin_popcount(masks[i]) != 3 || (masks[i] >> 28) != 0)
             return QS_ERR_STATE;
-    for (uint64_t b = 0; b < s->p.n_low; ++b) {
+    for (uint64_t b = 0; b < d.n_low; ++b) {
         uint32_t w = low[b * 8 + 7];
         if (w != 0 && (w <= QS_LINK_START || w >= next))
             return QS_ERR_STATE;
     }
     CK(hipDeviceSynchronize());
-    memcpy(s->h_masks, masks, sizeof(uint32_t) * s->p.n_masks);
-    s->seed = seed;
-    set_hashpar(s);
-    CK(hipMemcpy(s->d_low, low, (size_t)s->p.n_low * 32u, hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->d_high, high, (size_t)n_high * 128u, hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->d_masks, masks, sizeof(uint32_t) * s->p.n_masks, hipMemcpyHostToDevice));
-    CK(hipMemset(s->d_longbm, 0, sizeof(uint32_t) * (size_t)((s->p.n_low + 31) / 32)));
-    int rc = reset_state(s, next);
+    memcpy(s->h_masks + d.masks_off, masks, sizeof(uint32_t) * d.n_masks);
+    s->seeds[sid] = seed;
+    set_hash(d, seed);
+    CK(hipMemcpy(s->d_low + d.low_off, low, (size_t)d.n_low * 32u, hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->d_high + d.high_off, high, (size_t)n_high * 128u, hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->d_masks + d.masks_off, masks, sizeof(uint32_t) * d.n_masks, hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->d_desc + sid, &d, sizeof(qs_sdesc), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->d_next + sid, &next, sizeof(uint32_t), hipMemcpyHostToDevice));
+    CK(hipMemset(s->d_longbm, 0, s->bm_words * 4));
+    CK(hipDeviceSynchronize());
+    return QS_OK;
+}
+
+extern "C" int qs_export_state(qs_sketch *s, uint32_t *low, uint32_t *high, uint32_t *masks,
+                               uint32_t *seed, uint32_t *next)
+{
+    if (!s || s->nsk != 1)
+        return QS_ERR_PARAM;
+    return export_at(s, 0, low, high, masks, seed, next);
+}
+
+extern "C" int qs_import_state(qs_sketch *s, const uint32_t *low, const uint32_t *high,
+                               const uint32_t *masks, uint32_t seed, uint32_t next)
+{
+    if (!s || s->nsk != 1)
+        return QS_ERR_PARAM;
+    int rc = import_at(s, 0, low, high, masks, seed, next);
     if (rc)
         return rc;
-    CK(hipDeviceSynchronize());
     s->sticky_err = 0;
     s->keys_inserted = 0;
-    return QS_OK;
+    return reset_counts(s);
+}
+
+extern "C" int qs_multi_export_state(qs_sketch *s, uint32_t sketch, uint32_t *low, uint32_t *high,
+                                     uint32_t *masks, uint32_t *seed, uint32_t *next)
+{
+    return export_at(s, sketch, low, high, masks, seed, next);
+}
+
+extern "C" int qs_multi_import_state(qs_sketch *s, uint32_t sketch, const uint32_t *low, const uint32_t *high,
+                                     const uint32_t *masks, uint32_t seed, uint32_t next)
+{
+    return import_at(s, sketch, low, high, masks, seed, next);
 }
 
 extern "C" int qs_set_timing(qs_sketch *s, int enable)
@@ -652,18 +1007,26 @@ extern "C" int qs_get_stats(qs_sketch *s, qs_stats *st)
     qs_dev_state h;
     CK(hipDeviceSynchronize());
     CK(hipMemcpy(&h, s->d_st, sizeof(h), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> nx(s->nsk);
+    CK(hipMemcpy(nx.data(), s->d_next, sizeof(uint32_t) * s->nsk, hipMemcpyDeviceToHost));
     memset(st, 0, sizeof(*st));
-    st->n_low = s->p.n_low;
-    st->n_high = s->p.n_high;
-    st->n_masks = s->p.n_masks;
-    st->bytes_low = s->p.n_low * 32u;
-    st->bytes_high = s->p.n_high * 128u;
+    for (uint32_t i = 0; i < s->nsk; ++i) {
+        st->n_low += s->ps[i].n_low;
+        st->n_high += s->ps[i].n_high;
+        st->n_masks += s->ps[i].n_masks;
+        st->linked_buckets += nx[i] - (QS_LINK_START + 1);
+    }
+    st->bytes_low = st->n_low * 32u;
+    st->bytes_high = st->n_high * 128u;
     st->keys_inserted = s->keys_inserted;
     st->chunks = s->chunks;
-    st->chunks_with_fixup = s->chunks_fix;
-    st->fixup_keys = s->fix_keys;
-    st->next_id = h.next;
-    st->linked_buckets = h.next - (QS_LINK_START + 1);
-    st->seed = s->seed;
+    st->chunks_with_fixup = h.chunks_ord;
+    st->fixup_keys = h.replay_keys;
+    st->ordered_segments = h.ord_segments;
+    st->long_segments = h.long_segments;
+    st->next_id = nx[0];
+    st->seed = s->h_desc[0].seed;
+    st->n_sketches = s->nsk;
+    st->pool_error = h.err ? 1u : 0u;
     return QS_OK;
 }

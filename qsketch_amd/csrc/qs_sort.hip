@@ -28,11 +28,14 @@ size_t qs_sort_hist_words(uint32_t max_count)
 
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const uint32_t *__restrict__ keys,
                                                             uint32_t count, uint32_t ntiles,
-                                                            int shift, uint32_t *__restrict__ hist)
+                                                            int shift, uint32_t *__restrict__ hist,
+                                                            const uint32_t *__restrict__ d_count)
 {
     __shared__ uint32_t s_cnt[256];
     const uint32_t t = threadIdx.x;
     const uint32_t tile = blockIdx.x;
+    if (d_count)
+        count = *d_count < count ? *d_count : count;
     s_cnt[t] = 0;
     __syncthreads();
     const uint32_t base = tile * SORT_TILE;
@@ -101,9 +104,12 @@ __global__ __launch_bounds__(1024) void k_sort_scan_digit(uint32_t *__restrict__
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t count,
     uint32_t ntiles, int shift, const uint32_t *__restrict__ hist,
-    const uint32_t *__restrict__ totals, uint32_t *__restrict__ kout, uint32_t *__restrict__ vout)
+    const uint32_t *__restrict__ totals, uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+    const uint32_t *__restrict__ d_count)
 {
     __shared__ uint32_t s_base[256];
+    if (d_count)
+        count = *d_count < count ? *d_count : count;
     __shared__ uint32_t s_cnt[SORT_WAVES][256];
     __shared__ uint32_t s_tmp[SORT_WAVES];
     const uint32_t t = threadIdx.x;
@@ -182,11 +188,9 @@ void qs_scan_digits(uint32_t *hist, uint32_t ntiles, uint32_t nbins, uint32_t *t
     hipLaunchKernelGGL(k_sort_scan_digit, dim3(nbins), dim3(1024), 0, stream, hist, ntiles, totals);
 }
 
-int qs_sort_pairs(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist, uint32_t count,
-                  int key_bits, hipStream_t stream)
+static int sort_pairs_impl(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist, uint32_t count,
+                           const uint32_t *d_count, int key_bits, hipStream_t stream)
 {
-    if (count <= 1)
-        return 0;
     uint32_t ntiles = (count + SORT_TILE - 1) / SORT_TILE;
     uint32_t *totals = d_hist + (size_t)256 * ntiles;
     int passes = (key_bits + 7) / 8;
@@ -194,15 +198,31 @@ int qs_sort_pairs(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist, uint32
     for (int ps = 0; ps < passes; ++ps) {
         int shift = 8 * ps;
         hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, stream, keys[cur],
-                           count, ntiles, shift, d_hist);
+                           count, ntiles, shift, d_hist, d_count);
         hipLaunchKernelGGL(k_sort_scan_digit, dim3(256), dim3(1024), 0, stream, d_hist, ntiles,
                            totals);
         hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, stream,
                            keys[cur], vals[cur], count, ntiles, shift, d_hist, totals,
-                           keys[cur ^ 1], vals[cur ^ 1]);
+                           keys[cur ^ 1], vals[cur ^ 1], d_count);
         cur ^= 1;
     }
     if (hipGetLastError() != hipSuccess)
         return -1;
     return cur;
+}
+
+int qs_sort_pairs(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist, uint32_t count,
+                  int key_bits, hipStream_t stream)
+{
+    if (count <= 1)
+        return 0;
+    return sort_pairs_impl(keys, vals, d_hist, count, nullptr, key_bits, stream);
+}
+
+int qs_sort_pairs_dev(uint32_t *keys[2], uint32_t *vals[2], uint32_t *d_hist,
+                      const uint32_t *d_count, uint32_t max_count, int key_bits, hipStream_t stream)
+{
+    if (max_count == 0)
+        return 0;
+    return sort_pairs_impl(keys, vals, d_hist, max_count, d_count, key_bits, stream);
 }

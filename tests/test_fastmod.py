@@ -13,3 +13,18 @@ def test_fastmod_equals_modulo():
     r = subprocess.run([EXE], stdout=subprocess.PIPE, text=True, timeout=300)
     assert r.returncode == 0, r.stdout
     assert "mismatches: 0" in r.stdout
+
+
+PACK = os.path.join(os.path.dirname(__file__), "native", "pack_check")
+
+
+def test_item_packing_fields_and_order_keys():
+    """Regression: a 32-bit stream-index field made the allocation-order key
+    decode (key >> 32 wraps to key >> 0 on the GPU) return the index instead of
+    the sketch."""
+    if not os.path.exists(PACK):
+        from qsketch_amd import build as b
+        b.build_native_tests()
+    r = subprocess.run([PACK], stdout=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    assert "pack violations: 0" in r.stdout
