@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--workload", choices=["plain", "multi"], default="plain",
                     help="plain: one sketch (default); multi: 4096 independent sketches in one batch")
     ap.add_argument("--sketches", type=int, default=4096, help="multi: number of sketches")
-    ap.add_argument("--no-tune", action="store_true", help="multi: skip the shape autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="skip the segment-pass shape autotuner")
     return ap.parse_args()
 
 
@@ -101,6 +101,7 @@ def main():
     counts = torch.empty_like(uniq)
     n_low = q.number_of_buckets(N, 8, a.wlf)
     sk = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, a.n_high, 32, 1, 1021, rng_seed=1)
+    tune = None if a.no_tune else sk.autotune(keys, reps=1)
     torch.cuda.synchronize()
 
     def step():
@@ -172,6 +173,8 @@ def main():
             "bytes_per_key": QUERY_BYTES_PER_KEY,
             "avg_launch_ms": round(q_ms / max(1, tim["query_kernel_launches"]), 4),
         },
+        "shape": list(sk.shape()),
+        "tuner_best": tune["best"] if tune else None,
         "fixup": {"keys_replayed_per_step": st["fixup_keys"],
                   "segments_replayed_per_step": st["ordered_segments"],
                   "long_segments_per_step": st["long_segments"],
@@ -240,8 +243,11 @@ def bench_multi(a):
     for _ in range(a.warmup):
         run_set()
     t_set = [run_set() for _ in range(a.steps)]
+    # per-kernel breakdown: the same launch sequence, uncaptured, with events
+    S.clear()
     S.set_timing(True)
-    run_set()
+    S.insert_async(keys, offs, d_offs, 0)
+    S.sync()
     tim = S.timing()
     S.set_timing(False)
 

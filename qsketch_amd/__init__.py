@@ -25,8 +25,10 @@ QS_ERR_POOL = -3
 QS_ERR_HIP = -4
 QS_ERR_STATE = -5
 QS_ERR_NODEVICE = -6
+QS_ERR_TABLE = -7
 QS_LITERAL = 0
 QS_PAPER = 1
+MLP_PARAMS = 8385
 
 # lib/default.cu:39-54 and src/main.cu:390-391
 NEXT_LEVEL_ID_START = 1024
@@ -119,6 +121,25 @@ _SIGS = {
                                           C.POINTER(C.c_void_p)]),
     "qs_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "qs_graph_destroy": (None, [C.c_void_p]),
+    # learned heavy-hitter form (include/qsketch_learned.h)
+    "qs_learned_create": (C.c_int, [C.POINTER(qs_params), C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "qs_learned_destroy": (None, [C.c_void_p]),
+    "qs_learned_sketch": (C.c_void_p, [C.c_void_p]),
+    "qs_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "qs_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "qs_mlp_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                              C.c_float, C.POINTER(C.c_float), C.c_void_p]),
+    "qs_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                              C.c_void_p]),
+    "qs_mlp_score": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]),
+    "qs_learned_set_threshold": (C.c_int, [C.c_void_p, C.c_float]),
+    "qs_learned_threshold": (C.c_float, [C.c_void_p]),
+    "qs_learned_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "qs_learned_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "qs_learned_clear": (C.c_int, [C.c_void_p]),
+    "qs_learned_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "qs_learned_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64)]),
     "qs_set_insert_shape": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "qs_get_insert_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "qs_autotune_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
@@ -266,6 +287,28 @@ class Sketch_GPU_Sub_Warp_Level:
 
     def sync(self) -> None:
         check(_lib().qs_sync(self._h), "qs_sync")
+
+    def set_shape(self, width: int, items_per_lane: int) -> None:
+        """Segment-pass shape: sub-wavefront width x items per lane ((0, 0) = default)."""
+        check(_lib().qs_set_insert_shape(self._h, int(width), int(items_per_lane)), "qs_set_insert_shape")
+
+    def shape(self):
+        w, i = C.c_uint32(), C.c_uint32()
+        check(_lib().qs_get_insert_shape(self._h, C.byref(w), C.byref(i)), "qs_get_insert_shape")
+        return w.value, i.value
+
+    def autotune(self, keys, n: int | None = None, reps: int = 2) -> dict:
+        """Time all 16 segment-pass shapes on a twin sketch with these keys; keep the fastest."""
+        import numpy as np
+        n = keys.numel() if n is None else int(n)
+        o = np.array([0, n], dtype=np.uint64)
+        r = qs_tune_result()
+        rc = _lib().qs_autotune_insert(self._h, _ptr(keys), o.ctypes.data, None, int(reps), C.byref(r))
+        table = [{"width": r.e[i].width, "items_per_lane": r.e[i].items_per_lane,
+                  "seg_ms": round(r.e[i].seg_ms, 4), "insert_ms": round(r.e[i].insert_ms, 4)}
+                 for i in range(r.n)]
+        check(rc, "qs_autotune_insert")
+        return {"best": table[r.best], "consistent": bool(r.consistent), "table": table}
 
     def capture_insert(self, keys, n: int | None = None) -> "Graph":
         """Record the insert of n keys at this address into a hipGraph."""
@@ -457,6 +500,129 @@ class SketchSet:
     def destroy(self) -> None:
         if getattr(self, "_h", None):
             _lib().qs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class LearnedSketch:
+    """Learned heavy-hitter form (include/qsketch_learned.h): an MLP routes the
+    keys it scores at or above a threshold to an exact GPU table; the others go,
+    in stream order, to a plain sketch with parameters `params`."""
+
+    def __init__(self, params: qs_params, table_log2: int = 17, init_seed: int = 1):
+        h = C.c_void_p()
+        check(_lib().qs_learned_create(C.byref(params), int(table_log2), int(init_seed), C.byref(h)),
+              "qs_learned_create")
+        self._h = h
+        self.params = params
+        self.key_bytes = int(params.key_bytes)
+        self.table_log2 = int(table_log2)
+
+    # -- network ---------------------------------------------------------------
+    def get_params(self):
+        import numpy as np
+        a = np.zeros(MLP_PARAMS, np.float32)
+        check(_lib().qs_mlp_get_params(self._h, a.ctypes.data), "qs_mlp_get_params")
+        return a
+
+    def set_params(self, a) -> None:
+        import numpy as np
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.size == MLP_PARAMS
+        check(_lib().qs_mlp_set_params(self._h, a.ctypes.data), "qs_mlp_set_params")
+
+    def grad(self, keys, targets):
+        """(loss, FP32 gradient) of a batch; no update."""
+        import numpy as np
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        check(_lib().qs_mlp_grad(self._h, _ptr(keys), _ptr(targets), keys.numel(), C.byref(loss), g.ctypes.data),
+              "qs_mlp_grad")
+        return loss.value, g
+
+    def step(self, keys, targets, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999,
+             eps: float = 1e-8, want_loss: bool = True):
+        loss = C.c_float()
+        check(_lib().qs_mlp_step(self._h, _ptr(keys), _ptr(targets), keys.numel(), lr, beta1, beta2, eps,
+                                 C.byref(loss) if want_loss else None, None), "qs_mlp_step")
+        return loss.value if want_loss else None
+
+    def score(self, keys, mfma: bool = True, out=None):
+        import torch
+        if out is None:
+            out = torch.empty(keys.numel(), dtype=torch.float32, device=keys.device)
+        check(_lib().qs_mlp_score(self._h, _ptr(keys), keys.numel(), _ptr(out), 1 if mfma else 0),
+              "qs_mlp_score")
+        return out
+
+    # -- routing ---------------------------------------------------------------
+    @property
+    def threshold(self) -> float:
+        return float(_lib().qs_learned_threshold(self._h))
+
+    @threshold.setter
+    def threshold(self, v: float) -> None:
+        check(_lib().qs_learned_set_threshold(self._h, float(v)), "qs_learned_set_threshold")
+
+    def insert(self, keys, n: int | None = None) -> int:
+        """Insert; returns how many keys went to the exact table."""
+        n = keys.numel() if n is None else int(n)
+        nh = C.c_uint64()
+        check(_lib().qs_learned_insert(self._h, _ptr(keys), n, C.byref(nh)), "qs_learned_insert")
+        return nh.value
+
+    def search(self, keys, n: int | None = None, counts=None):
+        import torch
+        n = keys.numel() if n is None else int(n)
+        if counts is None:
+            counts = torch.empty(n, dtype=torch.int32, device=keys.device)
+        check(_lib().qs_learned_query(self._h, _ptr(keys), n, _ptr(counts)), "qs_learned_query")
+        return counts
+
+    query = search
+
+    def clear(self) -> None:
+        check(_lib().qs_learned_clear(self._h), "qs_learned_clear")
+
+    def table(self):
+        import numpy as np
+        cap = 1 << self.table_log2
+        k = np.zeros(cap, np.uint64)
+        c = np.zeros(cap, np.uint32)
+        used = C.c_uint64()
+        check(_lib().qs_learned_table(self._h, k.ctypes.data, c.ctypes.data, cap, C.byref(used)),
+              "qs_learned_table")
+        return k[:used.value], c[:used.value]
+
+    def memory(self) -> dict:
+        t, m, s = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(_lib().qs_learned_memory(self._h, C.byref(t), C.byref(m), C.byref(s)), "qs_learned_memory")
+        return {"table": t.value, "mlp": m.value, "sketch": s.value, "total": t.value + m.value + s.value}
+
+    def sketch_stats(self) -> dict:
+        st = qs_stats()
+        check(_lib().qs_get_stats(_lib().qs_learned_sketch(self._h), C.byref(st)), "qs_get_stats")
+        return {f: getattr(st, f) for f, _ in qs_stats._fields_}
+
+    def export_sketch_state(self):
+        import numpy as np
+        p = self.params
+        low = np.zeros(p.n_low * 8, dtype=np.uint32)
+        high = np.zeros(p.n_high * 32, dtype=np.uint32)
+        masks = np.zeros(p.n_masks, dtype=np.uint32)
+        seed, nxt = C.c_uint32(), C.c_uint32()
+        check(_lib().qs_export_state(_lib().qs_learned_sketch(self._h), low.ctypes.data, high.ctypes.data,
+                                     masks.ctypes.data, C.addressof(seed), C.addressof(nxt)), "qs_export_state")
+        return {"low": low, "high": high, "masks": masks, "seed": seed.value, "next": nxt.value}
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().qs_learned_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -144,6 +144,7 @@ extern "C" const char *qs_strerror(int code)
     case QS_ERR_HIP: return "HIP runtime error";
     case QS_ERR_STATE: return "malformed state";
     case QS_ERR_NODEVICE: return "no usable GPU";
+    case -7: return "exact table full (learned form: more routed keys than slots)";
     default: return "unknown error";
     }
 }
