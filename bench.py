@@ -39,8 +39,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24,
                     help="log2 of the stream prefix the CPU baseline inserts")
-    ap.add_argument("--workload", choices=["plain", "multi"], default="plain",
-                    help="plain: one sketch (default); multi: 4096 independent sketches in one batch")
+    ap.add_argument("--workload", choices=["plain", "multi", "learned"], default="plain",
+                    help="plain: one sketch (default); multi: 4096 independent sketches in one batch; "
+                         "learned: MLP-routed heavy hitters + sketch vs plain sketch at equal memory")
+    ap.add_argument("--mem-scales", type=str, default="1,0.25,0.0625",
+                    help="learned: sketch memory budgets as fractions of the sizing rule's")
+    ap.add_argument("--train-steps", type=int, default=3000, help="learned: Adam steps of 4096 pairs")
     ap.add_argument("--sketches", type=int, default=4096, help="multi: number of sketches")
     ap.add_argument("--no-tune", action="store_true", help="skip the segment-pass shape autotuner")
     return ap.parse_args()
@@ -83,6 +87,8 @@ def main():
         sys.exit(2)
     if a.workload == "multi":
         return bench_multi(a)
+    if a.workload == "learned":
+        return bench_learned(a)
     import torch
     import qsketch_amd as q
 
@@ -318,6 +324,168 @@ def bench_multi(a):
         "linked_buckets": st["linked_buckets"],
     }
     g.destroy()
+    print(json.dumps(out), flush=True)
+
+
+def bench_learned(a):
+    """Learned heavy-hitter form against the plain sketch at equal total memory.
+
+    Stream: 2^28 u64 keys, Zipf 1.1 over a 2^24-key universe; the upper 16 bits
+    of a key hold clip(round(log2(rank + 1)) + N(0, 1.5), 0, 31), the lower 48
+    bits are random (one key per rank).  The network is trained on what a plain
+    sketch estimated for the distinct keys of the first 2^26 items (target
+    log2(1 + estimate)); the threshold admits at most 2^16 of those keys to the
+    exact table.  Errors are measured over every distinct key of the stream
+    against exact counts (sort + reduce-by-key)."""
+    import numpy as np
+    import torch
+    import qsketch_amd as q
+    from qsketch_amd import accuracy as acc
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    N = 1 << a.log2n
+    U = 1 << 24
+    W = min(N, 1 << 26)
+    TOP = 1 << 16
+    TABLE_LOG2 = 19
+    z = q.Zipf(1.1, U)
+    keys = z.keys_u64_noisy(seed=1, n=N, sigma=1.5)
+    n_low0 = q.number_of_buckets(N, 8, a.wlf)
+    n_high = a.n_high
+
+    # 1. training pairs from a plain sketch over the first window
+    p0 = q.make_params(n_low0, n_high, key_bytes=8, rng_seed=1)
+    sk0 = q.Sketch_GPU_Sub_Warp_Level(n_low0, 8, n_high, 32, key_bytes=8, rng_seed=1)
+    sk0.insert(keys[:W])
+    wk = torch.unique(keys[:W])
+    west = sk0.search(wk).to(torch.int64) & 0xFFFFFFFF
+    targets = torch.log2(1.0 + west.to(torch.float32)).contiguous()
+    sk0.destroy()
+    del sk0
+
+    # 2. train (hand-written forward/backward + Adam) on shuffled pairs
+    L0 = q.LearnedSketch(p0, table_log2=TABLE_LOG2, init_seed=7)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(3)
+    B = 4096
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(a.train_steps):
+        idx = torch.randint(0, wk.numel(), (B,), device=dev, generator=gen)
+        want = it % 100 == 0 or it == a.train_steps - 1
+        l = L0.step(wk[idx].contiguous(), targets[idx].contiguous(), lr=2e-3, want_loss=want)
+        if want:
+            losses.append(round(l, 5))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    params = L0.get_params()
+    # threshold: the largest score set among the window's keys that holds <= TOP keys
+    # (scores tie wherever the network ignores the random low bits)
+    sc = L0.score(wk)
+    srt = torch.sort(sc, descending=True).values
+    thr = float(srt[min(TOP, srt.numel()) - 1])
+    if int((sc >= thr).sum()) > TOP:
+        above = sc[sc > thr]
+        thr = float(above.min()) if above.numel() else float("inf")
+    window_routed = int((sc >= thr).sum())
+    L0.destroy()
+    del L0
+
+    # 3. exact counts of the whole stream
+    uk, uc = torch.unique(keys, return_counts=True)
+
+    def time_it(setup, fn, reps):
+        ts = []
+        for _ in range(reps):
+            setup()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
+    rows = []
+    for scale in [float(x) for x in a.mem_scales.split(",")]:
+        n_low = q.largest_prime_at_most(max(1021, int(n_low0 * scale)))
+        plain = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, n_high, 32, key_bytes=8, rng_seed=1)
+        mem_plain = plain.memory_bytes()
+        # equal memory: the learned form's sketch gives up the table and network bytes
+        probe = q.LearnedSketch(q.make_params(n_low, n_high, key_bytes=8, rng_seed=1), table_log2=TABLE_LOG2)
+        extra = probe.memory()["table"] + probe.memory()["mlp"]
+        probe.destroy()
+        # (the bucket count stays prime, as the sizing rule's always is)
+        n_low_l = q.largest_prime_at_most(n_low - (extra + 31) // 32)
+        L = q.LearnedSketch(q.make_params(n_low_l, n_high, key_bytes=8, rng_seed=1), table_log2=TABLE_LOG2)
+        L.set_params(params)
+        L.threshold = thr
+        mem_learned = L.memory()["total"]
+
+        plain.insert(keys)  # warm-up (scratch allocation)
+        L.insert(keys)
+        t_plain = time_it(plain.clear, lambda: plain.insert(keys), a.steps)
+        t_learned = time_it(L.clear, lambda: L.insert(keys), a.steps)
+        L.clear()
+        n_heavy = L.insert(keys)
+        plain.clear()
+        plain.insert(keys)
+        est_p = plain.search(uk)
+        est_l = L.search(uk)
+        torch.cuda.synchronize()
+        sbuf = torch.empty(N, dtype=torch.float32, device=dev)
+        t_score = time_it(lambda: None, lambda: L.score(keys, out=sbuf), a.steps)
+        del sbuf
+        tq_plain = time_it(lambda: None, lambda: plain.search(uk, uk.numel(), est_p), a.steps)
+        tq_learned = time_it(lambda: None, lambda: L.search(uk, uk.numel(), est_l), a.steps)
+        mp = acc.metrics(est_p, uc)
+        ml = acc.metrics(est_l, uc)
+        tk, _ = L.table()
+        rows.append({
+            "mem_scale": scale, "n_low_plain": n_low, "n_low_learned": int(n_low_l),
+            "memory_plain": mem_plain, "memory_learned": mem_learned,
+            "are_plain": mp["are"], "are_learned": ml["are"],
+            "are_reduction": round(1 - ml["are"] / mp["are"], 4) if mp["are"] > 0 else None,
+            "aae_plain": mp["aae"], "aae_learned": ml["aae"],
+            "aae_reduction": round(1 - ml["aae"] / mp["aae"], 4) if mp["aae"] > 0 else None,
+            "under_estimates": [mp["under_estimates"], ml["under_estimates"]],
+            "insert_gkeys_s_plain": round(N / t_plain / 1e9, 3),
+            "insert_gkeys_s_learned": round(N / t_learned / 1e9, 3),
+            "insert_throughput_loss": round(1 - t_plain / t_learned, 4),
+            "query_gkeys_s_plain": round(uk.numel() / tq_plain / 1e9, 3),
+            "query_gkeys_s_learned": round(uk.numel() / tq_learned / 1e9, 3),
+            "heavy_fraction_of_stream": round(n_heavy / N, 4), "table_keys": int(tk.size),
+            "mlp_forward_gkeys_s": round(N / t_score / 1e9, 3),
+            "mlp_forward_tflops": round(N * 2 * (64 * 64 * 2 + 64) / t_score / 1e12, 2),
+        })
+        plain.destroy()
+        L.destroy()
+
+    main_row = rows[0]
+    out = {
+        "metric": "learned heavy-hitter form: ARE reduction at equal memory and insert Gkeys/s vs plain",
+        "value": main_row["insert_gkeys_s_learned"],
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": 1,
+        "ms_per_step": round(N / main_row["insert_gkeys_s_learned"] / 1e6, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 keys; bf16 MLP with fp32 accumulation",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2^{a.log2n} u64 keys Zipf 1.1 over 2^24, upper 16 bits = clip(round(log2(rank+1)) "
+                        f"+ N(0,1.5), 0, 31); MLP 64-64-64-1 trained on plain-sketch estimates of the first "
+                        f"2^26 items; <= 2^16 window keys over the threshold; table 2^{TABLE_LOG2} slots",
+            "keys": N, "distinct": int(uk.numel()), "train_pairs": int(wk.numel()),
+        },
+        "training": {"steps": a.train_steps, "batch": B, "seconds": round(t_train, 3),
+                     "loss_curve_every_100": losses, "threshold": thr, "window_keys_routed": window_routed},
+        "budgets": rows,
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -214,6 +214,37 @@ def find_greatest_prime(x: int) -> int:
     return int(_lib().qs_find_greatest_prime(int(x)))
 
 
+def largest_prime_at_most(x: int) -> int:
+    """Largest prime <= x (deterministic Miller-Rabin for 64-bit x).  The bucket
+    count must stay prime: with the multiplicative hash k * s mod 2^w, an even
+    modulus lets low key bits steer the bucket choice."""
+    def is_prime(n: int) -> bool:
+        if n < 2:
+            return False
+        for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+            if n % p == 0:
+                return n == p
+        d, r = n - 1, 0
+        while d % 2 == 0:
+            d //= 2
+            r += 1
+        for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+            x_ = pow(a, d, n)
+            if x_ in (1, n - 1):
+                continue
+            for _ in range(r - 1):
+                x_ = x_ * x_ % n
+                if x_ == n - 1:
+                    break
+            else:
+                return False
+        return True
+    x = int(x)
+    while x >= 2 and not is_prime(x):
+        x -= 1
+    return x if x >= 2 else 0
+
+
 def number_of_buckets(insert_keys_sz: int, m: int = 8, factor: float = 0.5) -> int:
     """Sketch_GPU::number_of_buckets (device_sketch.h:371-376)."""
     if m != 8:

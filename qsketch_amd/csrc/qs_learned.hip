@@ -304,50 +304,65 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const KeyT<KB> *__restrict__ ke
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-struct mlp_regs {
-    bf16x4 a1[2][8], a2[2][8];
-    float bb1[2][16], bb2[2][16], w3[2][16];
-    float b3;
+// The whole network lives in LDS: the weight operands in MFMA A-operand order
+// (layer, row tile, K step, lane -> 4 BF16), so each MFMA reads its operand
+// with one conflict-free 8-byte LDS load, plus the biases and output weights.
+// Registers then hold only accumulators and activations, for several waves
+// per SIMD.
+struct mlp_lds {
+    bf16x4 a[2][2][8][64];
+    float b1[64], b2[64], w3[64], b3;
 };
 
-__device__ static inline void load_mlp_regs(mlp_regs &R, const uint16_t *__restrict__ pb,
-                                            const float *__restrict__ p)
+__device__ static inline void load_mlp(mlp_lds &S, const uint16_t *__restrict__ pb, const float *__restrict__ p)
 {
-    const uint32_t lane = threadIdx.x & 63u, h = lane >> 5, c = lane & 31u;
+    for (uint32_t e = threadIdx.x; e < 2u * 2u * 8u * 64u; e += blockDim.x) {
+        const uint32_t l = e & 63u, s = (e >> 6) & 7u, t = (e >> 9) & 1u, layer = e >> 10;
+        const uint32_t h = l >> 5, c = l & 31u;
+        const uint16_t *w = pb + (layer ? OFF_W2 : OFF_W1) + (32u * t + c) * 64u + 8u * s + 4u * h;
+        bf16x4 v;
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int s = 0; s < 8; ++s)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                R.a1[t][s][i] = (short)pb[OFF_W1 + (32 * t + c) * 64 + 8 * s + 4 * h + i];
-                R.a2[t][s][i] = (short)pb[OFF_W2 + (32 * t + c) * 64 + 8 * s + 4 * h + i];
-            }
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const uint32_t row = 32 * t + 8 * (r / 4) + 4 * h + (r % 4);
-            R.bb1[t][r] = p[OFF_B1 + row];
-            R.bb2[t][r] = p[OFF_B2 + row];
-            R.w3[t][r] = bf2f(pb[OFF_W3 + row]);
-        }
-    R.b3 = p[OFF_B3];
+        for (int i = 0; i < 4; ++i)
+            v[i] = (short)w[i];
+        S.a[layer][t][s][l] = v;
+    }
+    for (uint32_t e = threadIdx.x; e < 64u; e += blockDim.x) {
+        S.b1[e] = p[OFF_B1 + e];
+        S.b2[e] = p[OFF_B2 + e];
+        S.w3[e] = bf2f(pb[OFF_W3 + e]);
+    }
+    if (threadIdx.x == 0)
+        S.b3 = p[OFF_B3];
+    __syncthreads();
+}
+
+// D row held by register r of row tile t in this lane
+__device__ static inline uint32_t drow(int t, int r, uint32_t h)
+{
+    return 32u * t + 8u * (r / 4) + 4u * h + (r % 4);
 }
 
 // score of key x (lane l scores key l % 32; both half-waves return it)
-__device__ static inline float mlp_mfma(const mlp_regs &R, uint64_t x)
+__device__ static inline float mlp_mfma(const mlp_lds &S, uint64_t x)
 {
-    const uint32_t h = (threadIdx.x & 63u) >> 5;
-    f32x16 acc0 = {0}, acc1 = {0};
+    const uint32_t lane = threadIdx.x & 63u, h = lane >> 5;
+    // reload the network from LDS for every key tile instead of letting the
+    // compiler hoist it into (occupancy-limiting) registers
+    asm volatile("" ::: "memory");
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        acc0[r] = S.b1[drow(0, r, h)];
+        acc1[r] = S.b1[drow(1, r, h)];
+    }
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         bf16x4 b;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             b[i] = ((x >> (8 * s + 4 * h + i)) & 1ull) ? (short)0x3f80 : (short)0;
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(R.a1[0][s], b, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(R.a1[1][s], b, acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(S.a[0][0][s][lane], b, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(S.a[0][1][s][lane], b, acc1, 0, 0, 0);
     }
     bf16x4 hb[8];
 #pragma unroll
@@ -355,23 +370,26 @@ __device__ static inline float mlp_mfma(const mlp_regs &R, uint64_t x)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = 4 * (s % 4) + i;
-            const float v = (s < 4) ? acc0[r] + R.bb1[0][r] : acc1[r] + R.bb1[1][r];
-            hb[s][i] = (short)f2bf(fmaxf(v, 0.f));
+            hb[s][i] = (short)f2bf(fmaxf(s < 4 ? acc0[r] : acc1[r], 0.f));
         }
-    f32x16 c0 = {0}, c1 = {0};
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        acc0[r] = S.b2[drow(0, r, h)];
+        acc1[r] = S.b2[drow(1, r, h)];
+    }
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        c0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(R.a2[0][s], hb[s], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(R.a2[1][s], hb[s], c1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(S.a[1][0][s][lane], hb[s], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(S.a[1][1][s][lane], hb[s], acc1, 0, 0, 0);
     }
     float y = 0.f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        y = fmaf(R.w3[0][r], bf2f(f2bf(fmaxf(c0[r] + R.bb2[0][r], 0.f))), y);
-        y = fmaf(R.w3[1][r], bf2f(f2bf(fmaxf(c1[r] + R.bb2[1][r], 0.f))), y);
+        y = fmaf(S.w3[drow(0, r, h)], bf2f(f2bf(fmaxf(acc0[r], 0.f))), y);
+        y = fmaf(S.w3[drow(1, r, h)], bf2f(f2bf(fmaxf(acc1[r], 0.f))), y);
     }
     y += __shfl_xor(y, 32, 64);
-    return y + R.b3;
+    return y + S.b3;
 }
 
 template <int KB>
@@ -379,17 +397,22 @@ __global__ __launch_bounds__(256) void k_mlp_mfma(const KeyT<KB> *__restrict__ k
                                                   const uint16_t *__restrict__ pb, const float *__restrict__ p,
                                                   float *__restrict__ out)
 {
-    mlp_regs R;
-    load_mlp_regs(R, pb, p);
+    __shared__ mlp_lds S;
+    load_mlp(S, pb, p);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t gw = ((uint64_t)blockIdx.x * 256u + threadIdx.x) >> 6;
     const uint64_t nw = ((uint64_t)gridDim.x * 256u) >> 6;
+    uint64_t i = gw * 32u + (lane & 31u);
+    uint64_t x = i < n ? key64<KB>(keys, i) : 0ull;
     for (uint64_t base = gw * 32u; base < n; base += nw * 32u) {
-        const uint64_t i = base + (lane & 31u);
-        const bool valid = i < n;
-        const float y = mlp_mfma(R, valid ? key64<KB>(keys, i) : 0ull);
-        if (valid && lane < 32u)
+        // the next key is loaded while this one is scored
+        const uint64_t i2 = i + nw * 32u;
+        const uint64_t x2 = i2 < n ? key64<KB>(keys, i2) : 0ull;
+        const float y = mlp_mfma(S, x);
+        if (i < n && lane < 32u)
             out[i] = y;
+        i = i2;
+        x = x2;
     }
 }
 
@@ -463,17 +486,24 @@ __global__ __launch_bounds__(ROUTE_THREADS) void k_route(const KeyT<KB> *__restr
     }
     if (threadIdx.x == 0)
         s_light = 0;
-    __syncthreads();
-    mlp_regs R;
-    load_mlp_regs(R, pb, p);
+    __shared__ mlp_lds S;
+    load_mlp(S, pb, p);
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint64_t tb = (uint64_t)blockIdx.x * ROUTE_TILE;
     uint32_t light = 0;
+    uint64_t xn = 0;
+    {
+        const uint64_t i0 = tb + (uint64_t)wv * 32u + (lane & 31u);
+        xn = i0 < n ? key64<KB>(keys, i0) : 0ull;
+    }
     for (uint32_t st = 0; st < ROUTE_TILE / 128u; ++st) {
         const uint64_t i = tb + (uint64_t)(st * 4u + wv) * 32u + (lane & 31u);
         const bool valid = i < n;
-        const uint64_t x = valid ? key64<KB>(keys, i) : 0ull;
-        const float y = mlp_mfma(R, x);
+        const uint64_t x = xn;
+        // the next step's key is loaded while this one is scored
+        const uint64_t in = i + 128u;
+        xn = (st + 1 < ROUTE_TILE / 128u && in < n) ? key64<KB>(keys, in) : 0ull;
+        const float y = mlp_mfma(S, x);
         if (lane >= 32u || !valid)
             continue;
         const bool heavy = y >= thr && x != T_EMPTY;
@@ -545,8 +575,8 @@ __global__ __launch_bounds__(256) void k_route_query(const KeyT<KB> *__restrict_
                                                      const uint32_t *__restrict__ tcnt, uint32_t tlog2,
                                                      uint32_t *__restrict__ out)
 {
-    mlp_regs R;
-    load_mlp_regs(R, pb, p);
+    __shared__ mlp_lds S;
+    load_mlp(S, pb, p);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t gw = ((uint64_t)blockIdx.x * 256u + threadIdx.x) >> 6;
     const uint64_t nw = ((uint64_t)gridDim.x * 256u) >> 6;
@@ -554,7 +584,7 @@ __global__ __launch_bounds__(256) void k_route_query(const KeyT<KB> *__restrict_
         const uint64_t i = base + (lane & 31u);
         const bool valid = i < n;
         const uint64_t x = valid ? key64<KB>(keys, i) : 0ull;
-        const float y = mlp_mfma(R, x);
+        const float y = mlp_mfma(S, x);
         if (lane < 32u && valid && y >= thr && x != T_EMPTY) {
             const uint32_t s = t_find(tkeys, tlog2, x);
             out[i] = s == T_NONE ? 0u : tcnt[s];

@@ -137,3 +137,13 @@ def test_noisy_u64_keys_encode_log_rank():
     assert abs(d.mean()) < 0.5 and 0.8 < d[(base > 4) & (base < 26)].std() < 2.2
     # the same rank always maps to the same key
     assert np.unique(k).size == np.unique(z.ranks_host(11, 1 << 16)).size
+
+
+def test_largest_prime_at_most():
+    import qsketch_amd as q
+    from oracle import pyoracle as po
+    assert [q.largest_prime_at_most(x) for x in (2, 3, 4, 10, 1021, 1022, 201129416)] == \
+        [2, 3, 3, 7, 1021, 1021, 201129389]
+    # agrees with the reference's prime table below its largest entry (3 * 2^27)
+    for k in range(2, 29):
+        assert q.largest_prime_at_most(1 << k) == po.find_greatest_prime(1 << k)
