@@ -24,6 +24,11 @@ import time
 HBM_PEAK_GBS = 8000.0
 INSERT_BYTES_PER_KEY = 68   # u32 key + 32 B bucket read + 32 B bucket write
 QUERY_BYTES_PER_KEY = 40    # u32 key + 32 B bucket read + 4 B estimate
+# HBM bytes per key the insert path moved on the default workload, from the
+# FETCH_SIZE / WRITE_SIZE counter runs calibrated with a 1 GiB copy
+# (profiles/insert_query_plain_zipf1.1_2e28_traffic.md)
+INSERT_TRAFFIC_BYTES_PER_KEY = 133.25
+DEFAULT_WORKLOAD = dict(log2n=28, log2u=26, alpha=1.1, wlf=0.5, n_high=1 << 18)
 
 
 def parse():
@@ -47,6 +52,8 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=3000, help="learned: Adam steps of 4096 pairs")
     ap.add_argument("--sketches", type=int, default=4096, help="multi: number of sketches")
     ap.add_argument("--no-tune", action="store_true", help="skip the segment-pass shape autotuner")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first copy 1 GiB device to device (a known-traffic kernel for counter calibration)")
     return ap.parse_args()
 
 
@@ -85,6 +92,14 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "designed (see DESIGN.md) but not built", "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.calib_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if a.workload == "multi":
         return bench_multi(a)
     if a.workload == "learned":
@@ -138,6 +153,23 @@ def main():
     q_achieved = tim["query_keys"] * QUERY_BYTES_PER_KEY / (q_ms * 1e-3) / 1e9 if q_ms > 0 else 0.0
     kern = {k: v for k, v in tim["kernels"].items() if k != "query" and v["launches"]}
     dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
+    # one "launch" of the insert path = the launch sequence of one chunk
+    chunks = max(1, kern.get("seg", {"launches": 1})["launches"])
+    keys_per_launch = ins_keys / chunks
+    default = all(getattr(a, k) == v for k, v in DEFAULT_WORKLOAD.items())
+    traffic = round(INSERT_TRAFFIC_BYTES_PER_KEY * keys_per_launch) if default else None
+    # the dominant kernel on its own: the radix scatter moves one 8-byte item
+    # in and out per key and pass (the first pass reads the 4-byte key)
+    sd = kern.get("sort_down")
+    dom_obj = None
+    if sd:
+        passes_per_chunk = sd["launches"] / chunks
+        item_bytes = (4 + 8 + (passes_per_chunk - 1) * 16) / passes_per_chunk
+        sd_avg = sd["ms"] / sd["launches"]
+        dom_obj = {"kernel": "k_bs_down (radix scatter)", "avg_launch_ms": round(sd_avg, 4),
+                   "algorithmic_bytes_per_launch": round(item_bytes * keys_per_launch),
+                   "achieved": round(item_bytes * keys_per_launch / (sd_avg * 1e-3) / 1e9, 1),
+                   "frac": round(item_bytes * keys_per_launch / (sd_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     out = {
         "metric": "insert & query Gkeys/s (% HBM roofline) at Zipf 1.1; ARE at equal memory",
         "value": round(value, 4),
@@ -166,11 +198,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "FETCH_SIZE/WRITE_SIZE runs of this workload calibrated by a 1 GiB copy, "
+                              "profiles/insert_query_plain_zipf1.1_2e28_traffic.md" if traffic else None,
+            "launch": "the launch sequence of one insert chunk (radix passes, k_seg, k_ordered, "
+                      "k_assign, k_tail, ...), timed with HIP events around each kernel on its stream",
             "bytes_per_key": INSERT_BYTES_PER_KEY,
-            "keys_per_launch": round(ins_keys / max(1, kern.get("seg", {"launches": 1})["launches"])),
+            "keys_per_launch": round(keys_per_launch),
+            "avg_launch_ms": round(ins_ms / chunks, 4),
             "path_ms_per_step": round(ins_ms / a.steps, 4),
             "dominant_kernel": dom,
+            "dominant": dom_obj,
             "kernel_ms_per_step": {k: round(v["ms"] / a.steps, 4) for k, v in kern.items()},
         },
         "roofline_query": {
@@ -184,7 +222,9 @@ def main():
         "fixup": {"keys_replayed_per_step": st["fixup_keys"],
                   "segments_replayed_per_step": st["ordered_segments"],
                   "long_segments_per_step": st["long_segments"],
-                  "linked_buckets": st["linked_buckets"]},
+                  "linked_buckets": st["linked_buckets"],
+                  "pool_occupancy": round(st["linked_buckets"] / (a.n_high - 1), 4),
+                  "allocator_retries": 0},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(keys, n_low, a.n_high, a.cpu_sample_log2)
