@@ -779,6 +779,49 @@ __device__ static inline uint32_t kth_set_bit(uint64_t m, uint32_t k)
 // Lane j (< 28) holds byte counter j; a batch of 64 inserts is transposed into
 // 28 ballots B_j, and the first insert that would see a selected byte > 128 is
 // the (129 - c_j)-th set bit of B_j, minimised over j.
+// A wavefront stages its allocations and newly long segments in LDS and
+// reserves their list slots TBUF at a time: one global atomic per segment on
+// a single counter would serialise (tens of thousands of segments replay in a
+// chunk that links the hot buckets).
+#define TBUF 8u
+
+__device__ static inline void flush_trig(qs_trig (*s_tr)[TBUF], uint32_t wv, uint32_t lane, uint32_t &ntb,
+                                         qs_trig *__restrict__ trig, uint32_t trig_cap,
+                                         qs_dev_state *__restrict__ st)
+{
+    if (!ntb)
+        return;
+    uint32_t base = 0;
+    if (lane == 0)
+        base = atomicAdd(&st->n_trig, ntb);
+    base = __shfl(base, 0, 64);
+    for (uint32_t e = 0; e < ntb; ++e) {
+        const uint32_t slot = base + e;
+        if (slot < trig_cap) {
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(&s_tr[wv][e]);
+            uint32_t *dst = reinterpret_cast<uint32_t *>(trig + slot);
+            if (lane < sizeof(qs_trig) / 4u)
+                dst[lane] = src[lane];
+        } else if (lane == 0) {
+            atomicOr(&st->err, 1u); // more allocations than the pools hold
+        }
+    }
+    ntb = 0;
+}
+
+__device__ static inline void flush_long(uint32_t (*s_ln)[TBUF], uint32_t wv, uint32_t lane, uint32_t &nlb,
+                                         uint32_t *__restrict__ lng, qs_dev_state *__restrict__ st)
+{
+    if (!nlb)
+        return;
+    uint32_t base = 0;
+    if (lane == 0)
+        base = atomicAdd(&st->n_long, nlb);
+    base = __shfl(base, 0, 64);
+    if (lane < nlb)
+        lng[base + lane] = s_ln[wv][lane];
+    nlb = 0;
+}
 __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT, uint32_t n,
                                                  const uint32_t *__restrict__ ord,
                                                  const qs_sdesc *__restrict__ D,
@@ -789,10 +832,14 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
                                                  qs_trig *__restrict__ trig, uint32_t trig_cap,
                                                  qs_dev_state *__restrict__ st)
 {
-    const uint32_t lane = threadIdx.x & 63u;
+    __shared__ qs_trig s_tr[4][TBUF];
+    __shared__ uint32_t s_ln[4][TBUF];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t gw = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * (blockDim.x / 64u);
     const uint32_t n_ord = st->n_ord;
+    uint32_t ntb = 0, nlb = 0;
+    unsigned long long replayed_all = 0;
     for (uint32_t h = gw; h < n_ord; h += nw) {
         const uint32_t p0 = ord[h];
         const uint64_t cell = it_cell(IT[p0], pk);
@@ -863,37 +910,43 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
                       ((uint32_t)__shfl(c, (int)(4u * lane + 3u) & 63, 64) << 24);
         if (lane < 7u)
             bk[lane] = w4;
-        if (lane == 0)
-            atomicAdd(&st->replay_keys, (unsigned long long)replayed);
+        replayed_all += replayed;
         if (fired) {
-            uint32_t slot = 0;
+            // stage the allocation; its list slot is reserved later
+            if (lane < 32u)
+                s_tr[wv][ntb].hc[lane] = hc;
             if (lane == 0) {
-                slot = atomicAdd(&st->n_trig, 1u);
-                // the segment continues after the batch of its allocation: make
-                // sure the tail kernel sees it
-                if (more && tail_start < n) {
-                    const uint64_t bit = d.bm_off + b;
-                    uint32_t old = atomicOr(longbm + (bit >> 5), 1u << (bit & 31u));
-                    if (!(old & (1u << (bit & 31u))))
-                        lng[atomicAdd(&st->n_long, 1u)] = p0;
-                }
+                s_tr[wv][ntb].idx = t_idx;
+                s_tr[wv][ntb].bucket = b;
+                s_tr[wv][ntb].tail_start = tail_start;
+                s_tr[wv][ntb].sid = sid;
             }
-            slot = __shfl(slot, 0, 64);
-            if (slot < trig_cap) {
-                qs_trig *tr = trig + slot;
+            ++ntb;
+            // the segment continues after the batch of its allocation: make
+            // sure the tail kernel sees it
+            if (more && tail_start < n) {
+                uint32_t newly = 0;
                 if (lane == 0) {
-                    tr->idx = t_idx;
-                    tr->bucket = b;
-                    tr->tail_start = tail_start;
-                    tr->sid = sid;
+                    const uint64_t bit = d.bm_off + b;
+                    const uint32_t old = atomicOr(longbm + (bit >> 5), 1u << (bit & 31u));
+                    newly = (old & (1u << (bit & 31u))) ? 0u : 1u;
                 }
-                if (lane < 32u)
-                    tr->hc[lane] = hc;
-            } else if (lane == 0) {
-                atomicOr(&st->err, 1u); // more allocations than the pools hold
+                if (__shfl((int)newly, 0, 64)) {
+                    if (lane == 0)
+                        s_ln[wv][nlb] = p0;
+                    ++nlb;
+                }
             }
+            if (ntb == TBUF)
+                flush_trig(s_tr, wv, lane, ntb, trig, trig_cap, st);
+            if (nlb == TBUF)
+                flush_long(s_ln, wv, lane, nlb, lng, st);
         }
     }
+    flush_trig(s_tr, wv, lane, ntb, trig, trig_cap, st);
+    flush_long(s_ln, wv, lane, nlb, lng, st);
+    if (lane == 0 && replayed_all)
+        atomicAdd(&st->replay_keys, replayed_all);
 }
 
 // trigger sort key: sketch << ib | stream index (fits 32 bits by construction)
