@@ -185,11 +185,13 @@ class OracleSketch:
     def masks(self) -> np.ndarray:
         return np.ctypeslib.as_array(self.s.masks, shape=(self.s.n_masks,)).copy()
 
-    def low(self) -> np.ndarray:
-        return np.ctypeslib.as_array(self.s.low, shape=(self.s.n_low * 8,)).copy()
+    def low(self, copy: bool = True) -> np.ndarray:
+        a = np.ctypeslib.as_array(self.s.low, shape=(self.s.n_low * 8,))
+        return a.copy() if copy else a
 
-    def high(self) -> np.ndarray:
-        return np.ctypeslib.as_array(self.s.high, shape=(self.s.n_high * 32,)).copy()
+    def high(self, copy: bool = True) -> np.ndarray:
+        a = np.ctypeslib.as_array(self.s.high, shape=(self.s.n_high * 32,))
+        return a.copy() if copy else a
 
 
 def exact_counts(keys, key_bytes=4):
