@@ -481,6 +481,16 @@ def bench_learned(a):
         tq_learned = time_it(lambda: None, lambda: L.search(uk, uk.numel(), est_l), a.steps)
         mp = acc.metrics(est_p, uc)
         ml = acc.metrics(est_l, uc)
+        # the reference's single-level sub-warp sketch (m = 8) at the same bytes
+        sl = q.Sketch_GPU_Sub_Warp(q.largest_prime_at_most(mem_plain // 32), 8, key_bytes=8, rng_seed=1)
+        sl.insert(keys)
+        t_sl = time_it(sl.clear, lambda: sl.insert(keys), a.steps)
+        sl.clear()
+        sl.insert(keys)
+        msl = acc.metrics(sl.search(uk), uc)
+        sl_row = {"n": sl.n, "memory": sl.memory_bytes(), "are": msl["are"], "aae": msl["aae"],
+                  "insert_gkeys_s": round(N / t_sl / 1e9, 3)}
+        sl.destroy()
         tk, _ = L.table()
         rows.append({
             "mem_scale": scale, "n_low_plain": n_low, "n_low_learned": int(n_low_l),
@@ -496,6 +506,7 @@ def bench_learned(a):
             "query_gkeys_s_plain": round(uk.numel() / tq_plain / 1e9, 3),
             "query_gkeys_s_learned": round(uk.numel() / tq_learned / 1e9, 3),
             "heavy_fraction_of_stream": round(n_heavy / N, 4), "table_keys": int(tk.size),
+            "single_level_equal_memory": sl_row,
             "mlp_forward_gkeys_s": round(N / t_score / 1e9, 3),
             "mlp_forward_tflops": round(N * 2 * (64 * 64 * 2 + 64) / t_score / 1e12, 2),
         })

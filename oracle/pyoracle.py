@@ -73,6 +73,11 @@ def lib():
         L.orq_query.argtypes = [C.POINTER(_Sk), vp, sz, C.POINTER(u32)]
         L.orq_query_mt.argtypes = [C.POINTER(_Sk), vp, sz, C.POINTER(u32), C.c_int]
         L.orq_exact_counts.argtypes = [vp, sz, u32, vp, C.POINTER(u32)]
+        L.orq_sl_create.argtypes = [C.POINTER(_Sl), u64, u32, u32, u32, u32, u64]
+        L.orq_sl_create.restype = C.c_int
+        L.orq_sl_destroy.argtypes = [C.POINTER(_Sl)]
+        L.orq_sl_insert.argtypes = [C.POINTER(_Sl), vp, sz]
+        L.orq_sl_query.argtypes = [C.POINTER(_Sl), vp, sz, C.POINTER(u32)]
         L.orq_exact_counts.restype = sz
         _lib = L
     return _lib
@@ -192,6 +197,51 @@ class OracleSketch:
     def high(self, copy: bool = True) -> np.ndarray:
         a = np.ctypeslib.as_array(self.s.high, shape=(self.s.n_high * 32,))
         return a.copy() if copy else a
+
+
+class _Sl(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("m", C.c_uint32), ("n_masks", C.c_uint32), ("key_bytes", C.c_uint32),
+                ("seed", C.c_uint32), ("masks", C.POINTER(C.c_uint32)), ("table", C.POINTER(C.c_uint32))]
+
+
+class OracleSingleLevel:
+    """Single-level sub-warp sketch (device_sketch_v2.h): one bucket of m u32
+    counters per key, 3 selected counters, count-min query."""
+
+    def __init__(self, n, m=8, n_masks=1021, seed_sz=1, key_bytes=4, rng_seed=1):
+        self.s = _Sl()
+        self.key_bytes = key_bytes
+        rc = lib().orq_sl_create(C.byref(self.s), int(n), int(m), int(n_masks), int(seed_sz), int(key_bytes),
+                                 int(rng_seed))
+        if rc:
+            raise RuntimeError(f"orq_sl_create failed: {rc}")
+
+    def __del__(self):
+        if getattr(self, "s", None) is not None and self.s.table:
+            lib().orq_sl_destroy(C.byref(self.s))
+
+    def _keys(self, keys):
+        return np.ascontiguousarray(keys, dtype=np.uint32 if self.key_bytes == 4 else np.uint64)
+
+    def insert(self, keys) -> None:
+        k = self._keys(keys)
+        lib().orq_sl_insert(C.byref(self.s), k.ctypes.data, k.size)
+
+    def query(self, keys) -> np.ndarray:
+        k = self._keys(keys)
+        out = np.zeros(k.size, dtype=np.uint32)
+        lib().orq_sl_query(C.byref(self.s), k.ctypes.data, k.size, _p32(out))
+        return out
+
+    @property
+    def seed(self) -> int:
+        return self.s.seed
+
+    def masks(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.s.masks, shape=(self.s.n_masks,)).copy()
+
+    def table(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.s.table, shape=(self.s.n * self.s.m,)).copy()
 
 
 def exact_counts(keys, key_bytes=4):

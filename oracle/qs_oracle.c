@@ -33,6 +33,11 @@
  *                          published accuracy (mean est/true on uniform keys)
  *   orq_insert_mt          bucket-range partition of orq_insert
  *                          pinned: byte-identical state to orq_insert
+ *   orq_sl_*               device_sketch_v2.h:67-283 (single-level sub-warp
+ *                          sketch), construction device_sketch_v1.h:254-273
+ *                          pinned: brute force against a naive Python model,
+ *                          the conservation invariant sum = 3N, and the
+ *                          textbook count-min bound (estimate >= true count)
  *   orq_exact_counts       host_sketch.h:15-63 (std::unordered_map counter)
  *                          pinned: equality with a Python dict counter
  *
@@ -611,4 +616,88 @@ size_t orq_exact_counts(const void *keys, size_t n, uint32_t key_bytes, void *un
     }
     free(a);
     return u;
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* single-level sub-warp sketch                                              */
+/* ------------------------------------------------------------------------ */
+int orq_sl_create(orq_sl *s, uint64_t n, uint32_t m, uint32_t n_masks, uint32_t seed_sz,
+                  uint32_t key_bytes, uint64_t rng_seed)
+{
+    if (!s || n == 0 || (m != 8 && m != 16 && m != 32) || n_masks == 0 ||
+        (key_bytes != 4 && key_bytes != 8))
+        return ORQ_ERR_ARG;
+    int8_t buf[128];
+    orq_minstd seed_eng, mask_eng;
+    orq_seed_buffer(rng_seed, 0, buf);
+    orq_minstd_seed_bytes(&seed_eng, buf, 128);
+    orq_seed_buffer(rng_seed, 1, buf);
+    orq_minstd_seed_bytes(&mask_eng, buf, 128);
+    uint32_t seed = 0;
+    for (uint32_t i = 0; i < seed_sz; ++i)
+        seed += orq_uniform_u32(&seed_eng, 1u, 0xffffffffu);
+    const uint32_t sz = (m + 7) / 8;
+    uint8_t *mb = (uint8_t *)calloc((size_t)n_masks * sz, 1);
+    s->masks = (uint32_t *)malloc(sizeof(uint32_t) * n_masks);
+    s->table = (uint32_t *)calloc((size_t)n * m, sizeof(uint32_t));
+    if (!mb || !s->masks || !s->table) {
+        free(mb);
+        free(s->masks);
+        free(s->table);
+        return ORQ_ERR_NOMEM;
+    }
+    orq_generate_hashmask(&mask_eng, n_masks, m, 3, 0, mb);
+    for (uint32_t i = 0; i < n_masks; ++i) {
+        uint32_t v = 0;
+        for (uint32_t b = 0; b < sz; ++b)
+            v |= (uint32_t)mb[(size_t)i * sz + b] << (8 * b);
+        s->masks[i] = v;
+    }
+    free(mb);
+    s->n = n;
+    s->m = m;
+    s->n_masks = n_masks;
+    s->key_bytes = key_bytes;
+    s->seed = seed;
+    return ORQ_OK;
+}
+
+void orq_sl_destroy(orq_sl *s)
+{
+    free(s->masks);
+    free(s->table);
+    s->masks = NULL;
+    s->table = NULL;
+}
+
+static inline uint64_t sl_hash(const orq_sl *s, const void *keys, size_t i)
+{
+    if (s->key_bytes == 4)
+        return (uint32_t)(((const uint32_t *)keys)[i] * s->seed);
+    return ((const uint64_t *)keys)[i] * (uint64_t)s->seed;
+}
+
+void orq_sl_insert(orq_sl *s, const void *keys, size_t n)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t h = sl_hash(s, keys, i);
+        uint32_t *b = s->table + (h % s->n) * s->m;
+        for (uint32_t M = s->masks[h % s->n_masks]; M; M &= M - 1)
+            b[__builtin_ctz(M)] += 1u;
+    }
+}
+
+void orq_sl_query(const orq_sl *s, const void *keys, size_t n, uint32_t *out)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t h = sl_hash(s, keys, i);
+        const uint32_t *b = s->table + (h % s->n) * s->m;
+        uint32_t v = 0xffffffffu;
+        for (uint32_t M = s->masks[h % s->n_masks]; M; M &= M - 1) {
+            const uint32_t c = b[__builtin_ctz(M)];
+            v = c < v ? c : v;
+        }
+        out[i] = v;
+    }
 }

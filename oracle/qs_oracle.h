@@ -92,6 +92,26 @@ void orq_query(const orq_sketch *s, const void *keys, size_t n, uint32_t *out);
 void orq_query_mt(const orq_sketch *s, const void *keys, size_t n, uint32_t *out,
                   int nthreads);
 
+/* ---- single-level sub-warp sketch (device_sketch_v2.h:67-283) ----------- */
+/* n buckets of m u32 counters (m in {8, 16, 32}); 1021 masks of 3 distinct
+ * bits in [0, m) from generate_hashmask(n_masks, m, 3, 0) (the constructor
+ * call of device_sketch_v1.h:267-273); insert adds 1 to each selected counter
+ * (v2:54-65), query takes their minimum (v2:155-190). */
+typedef struct {
+    uint64_t  n;
+    uint32_t  m;
+    uint32_t  n_masks;
+    uint32_t  key_bytes;
+    uint32_t  seed;
+    uint32_t *masks;   /* bits 0..m-1 */
+    uint32_t *table;   /* n * m counters */
+} orq_sl;
+int  orq_sl_create(orq_sl *s, uint64_t n, uint32_t m, uint32_t n_masks, uint32_t seed_sz,
+                   uint32_t key_bytes, uint64_t rng_seed);
+void orq_sl_destroy(orq_sl *s);
+void orq_sl_insert(orq_sl *s, const void *keys, size_t n);
+void orq_sl_query(const orq_sl *s, const void *keys, size_t n, uint32_t *out);
+
 /* ---- exact counting (sort + reduce-by-key) ------------------------------ */
 /* returns number of distinct keys; uniq/counts sized n by the caller */
 size_t orq_exact_counts(const void *keys, size_t n, uint32_t key_bytes,

@@ -140,6 +140,14 @@ _SIGS = {
     "qs_learned_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "qs_learned_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64)]),
+    # single-level sub-warp sketch (include/qsketch_single.h)
+    "qs_sl_create": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                               C.POINTER(C.c_void_p)]),
+    "qs_sl_destroy": (None, [C.c_void_p]),
+    "qs_sl_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "qs_sl_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "qs_sl_clear": (C.c_int, [C.c_void_p]),
+    "qs_sl_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "qs_set_insert_shape": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "qs_get_insert_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "qs_autotune_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
@@ -531,6 +539,60 @@ class SketchSet:
     def destroy(self) -> None:
         if getattr(self, "_h", None):
             _lib().qs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Sketch_GPU_Sub_Warp:
+    """Single-level sub-warp sketch (include/qsketch_single.h), mirroring the
+    reference's Sketch_GPU_Sub_Warp(n, m, seed_sz, n_hash_mask_table)
+    (device_sketch_v2.h:307-309): n buckets of m u32 counters, count-min query."""
+
+    def __init__(self, n: int, m: int = 8, seed_sz: int = SEED_SZ, n_hash_mask_table: int = HASH_MASK_TABLE_SZ,
+                 *, key_bytes: int = 4, rng_seed: int = 1):
+        h = C.c_void_p()
+        check(_lib().qs_sl_create(int(n), int(m), int(seed_sz), int(n_hash_mask_table), int(key_bytes),
+                                  int(rng_seed), C.byref(h)), "qs_sl_create")
+        self._h = h
+        self.n, self.m, self.n_masks, self.key_bytes = int(n), int(m), int(n_hash_mask_table), int(key_bytes)
+
+    def insert(self, keys, keys_sz: int | None = None) -> int:
+        n = keys.numel() if keys_sz is None else int(keys_sz)
+        check(_lib().qs_sl_insert(self._h, _ptr(keys), n), "qs_sl_insert")
+        return 0
+
+    def search(self, keys, keys_sz: int | None = None, counts=None):
+        import torch
+        n = keys.numel() if keys_sz is None else int(keys_sz)
+        if counts is None:
+            counts = torch.empty(n, dtype=torch.int32, device=keys.device)
+        check(_lib().qs_sl_query(self._h, _ptr(keys), n, _ptr(counts)), "qs_sl_query")
+        return counts
+
+    query = search
+
+    def clear(self) -> None:
+        check(_lib().qs_sl_clear(self._h), "qs_sl_clear")
+
+    def export_state(self):
+        import numpy as np
+        table = np.zeros(self.n * self.m, np.uint32)
+        masks = np.zeros(self.n_masks, np.uint32)
+        seed = C.c_uint32()
+        check(_lib().qs_sl_export(self._h, table.ctypes.data, masks.ctypes.data, C.addressof(seed)), "qs_sl_export")
+        return {"table": table, "masks": masks, "seed": seed.value}
+
+    def memory_bytes(self) -> int:
+        return self.n * self.m * 4
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().qs_sl_destroy(self._h)
             self._h = None
 
     def __del__(self):

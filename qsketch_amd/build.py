@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqsketch_amd.so")
-SOURCES = ["qs_sketch.hip", "qs_insert.hip", "qs_sort.hip", "qs_gen.hip", "qs_batched.hip", "qs_learned.hip"]
+SOURCES = ["qs_sketch.hip", "qs_insert.hip", "qs_sort.hip", "qs_gen.hip", "qs_learned.hip", "qs_single.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QS_OFFLOAD_ARCH", "gfx950")

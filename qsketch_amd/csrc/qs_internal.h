@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <random>
+
 #include "../../include/qsketch.h"
 
 #define QS_LINK_START 1024u
@@ -278,6 +280,13 @@ bool qs_seg_shape_ok(uint32_t w, uint32_t ipl);
 uint64_t qs_chunk_keys(const qs_sketch *s, const uint64_t *offs);
 void qs_free_scratch(qs_sketch *s);
 uint64_t qs_scratch_hist_words(uint64_t chunk);
+
+// the reference's seed and mask generation (qs_sketch.hip): the libstdc++
+// engine seeded from a 128-byte buffer of (rng_seed, stream), and
+// generate_hashmask's draw of 3 distinct bits in [0, m_bits - padding)
+void qs_seed_engine(std::default_random_engine &eng, uint64_t rng_seed, uint64_t stream);
+void qs_make_masks(std::default_random_engine &eng, uint32_t n, uint32_t m_bits, uint32_t padding,
+                   uint32_t *out);
 
 // timing hooks (qs_sketch.hip)
 struct qs_evpair {

@@ -154,7 +154,7 @@ extern "C" const char *qs_strerror(int code)
 // a SplitMix64 stream of (rng_seed, stream) so that a sketch is reproducible;
 // stream 0 feeds the seed engine (gpu_tool<uint>), stream 1 the mask engine
 // (cpu_tool<uint>).
-static void seed_engine(std::default_random_engine &eng, uint64_t rng_seed, uint64_t stream)
+void qs_seed_engine(std::default_random_engine &eng, uint64_t rng_seed, uint64_t stream)
 {
     char buf[128];
     uint64_t st = rng_seed ^ (stream * 0xD1B54A32D192ED03ull);
@@ -170,11 +170,13 @@ static void seed_engine(std::default_random_engine &eng, uint64_t rng_seed, uint
     eng.seed(sq);
 }
 
-// generate_hashmask(nullptr, n, 32, 3, 4) (lib/hashmask.cu:32-82): draw bit
-// positions uniformly in [0, 27] until three distinct ones are set.
-static void make_masks(std::default_random_engine &eng, uint32_t n, uint32_t *out)
+// generate_hashmask(nullptr, n, m_bits, 3, padding) (lib/hashmask.cu:32-82):
+// draw bit positions uniformly in [0, m_bits - padding - 1] until three
+// distinct ones are set.  The two-level sketch uses (32, 4), i.e. [0, 27].
+void qs_make_masks(std::default_random_engine &eng, uint32_t n, uint32_t m_bits, uint32_t padding,
+                   uint32_t *out)
 {
-    std::uniform_int_distribution<unsigned int> dis(0, 32 - 4 - 1);
+    std::uniform_int_distribution<unsigned int> dis(0, m_bits - padding - 1);
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t m = 0;
         uint32_t ones = 0;
@@ -418,14 +420,14 @@ static int create_set(const qs_params *ps, uint32_t nsk, qs_sketch **out)
         const qs_params &p = ps[i];
         // seeds (Device_Seed::resize, device_sketch.h:280-302) and masks (v3:558-562)
         std::default_random_engine seed_eng, mask_eng;
-        seed_engine(seed_eng, p.rng_seed, 0);
-        seed_engine(mask_eng, p.rng_seed, 1);
+        qs_seed_engine(seed_eng, p.rng_seed, 0);
+        qs_seed_engine(mask_eng, p.rng_seed, 1);
         uint32_t seed = 0;
         std::uniform_int_distribution<unsigned int> dis(1, std::numeric_limits<unsigned int>::max());
         for (uint32_t k = 0; k < p.seed_sz; ++k)
             seed += dis(seed_eng);
         s->seeds[i] = p.hash_seed ? p.hash_seed : seed;
-        make_masks(mask_eng, p.n_masks, s->h_masks + mask_w);
+        qs_make_masks(mask_eng, p.n_masks, 32, 4, s->h_masks + mask_w);
         qs_sdesc &d = s->h_desc[i];
         d.low_off = low_w;
         d.high_off = high_w;

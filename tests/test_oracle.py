@@ -374,3 +374,45 @@ def test_accuracy_matches_reference_published(wlf, lo, hi):
         true = cnt[np.searchsorted(uq, q)]
         pct = (s.query(q) / true * 100.0).mean()
         assert lo < pct < hi, (wlf, k, pct)
+
+
+# ---------------------------------------------------------------------------
+# single-level sub-warp sketch (device_sketch_v2.h)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("m", [8, 16, 32])
+def test_single_level_matches_naive_model(m):
+    rs = np.random.default_rng(m)
+    n = 13
+    o = po.OracleSingleLevel(n, m, 1021, rng_seed=5)
+    masks = o.masks()
+    assert all(bin(int(x)).count("1") == 3 and int(x) >> m == 0 for x in masks)
+    keys = rs.integers(0, 200, 3000).astype(np.uint32)
+    o.insert(keys)
+    seed = o.seed
+    table = np.zeros(n * m, dtype=np.int64)
+    for k in keys.tolist():
+        h = (k * seed) & 0xFFFFFFFF
+        b, M = h % n, int(masks[h % 1021])
+        for j in range(m):
+            if M >> j & 1:
+                table[b * m + j] += 1
+    assert np.array_equal(o.table(), table)
+    assert o.table().sum() == 3 * keys.size
+    uq, cnt = np.unique(keys, return_counts=True)
+    est = o.query(uq)
+    assert np.all(est >= cnt)
+    for k, e in zip(uq.tolist(), est.tolist()):
+        h = (k * seed) & 0xFFFFFFFF
+        b, M = h % n, int(masks[h % 1021])
+        assert e == min(table[b * m + j] for j in range(m) if M >> j & 1)
+
+
+def test_single_level_seed_and_masks_follow_the_two_level_engines():
+    # same engines and call pattern as the two-level sketch: identical seed; the
+    # masks draw from [0, m) instead of [0, 28)
+    o2 = po.OracleSketch(1021, 1024, 1021, rng_seed=9)
+    o1 = po.OracleSingleLevel(1021, 8, 1021, rng_seed=9)
+    assert o1.seed == o2.seed
+    m = po.Minstd(seed_bytes=po.seed_buffer(9, 1))
+    ref = m.hashmask(1021, 8, 3, 0)
+    assert np.array_equal(o1.masks(), ref.astype(np.uint32).reshape(-1))
