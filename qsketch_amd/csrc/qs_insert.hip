@@ -20,9 +20,10 @@
 //
 // Each insert is hashed once, in the first radix pass, into a 64-bit item
 // (bucket | sketch | mask index | stream index); later passes and kernels
-// only extract bit fields.  Digit ranks inside a wavefront come from an LDS
-// match (every lane ORs its lane bit into the slot of its digit and reads the
-// group mask back), which is stable and costs three LDS operations per item.
+// only extract bit fields.  Digit ranks inside a wavefront come from a ballot
+// match (one ballot per digit bit gives each lane the mask of lanes sharing
+// its digit), which is stable; the group's lowest lane advances the digit's
+// running count with one LDS atomic.
 //
 // Per chunk (a fixed launch sequence with no host synchronisation, so it can
 // be captured in a hipGraph; all counts stay on the device):
@@ -176,7 +177,7 @@ __device__ static inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_red
 }
 
 // Stable scatter of one tile.  Wave w ranks its 1024 consecutive items in 16
-// rounds: the LDS match gives the lanes sharing a digit, the lowest of them
+// rounds: the ballot match gives the lanes sharing a digit, the lowest of them
 // advances the wave's running count.  The tile is then staged digit-sorted in
 // LDS and written out as contiguous digit runs.
 template <int KB, bool FIRST, bool MULTI>
@@ -187,7 +188,6 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const uint32_t *__restrict__ totals, uint64_t *__restrict__ out)
 {
     __shared__ uint64_t s_item[OS_TILE];
-    __shared__ unsigned long long s_match[OS_WAVES][OS_BINS];
     __shared__ uint32_t s_wh[OS_WAVES][OS_BINS];
     __shared__ uint32_t s_ls[OS_BINS];
     __shared__ uint32_t s_goff[OS_BINS];
@@ -196,10 +196,8 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const uint32_t tile = blockIdx.x;
     const uint32_t tb = tile * OS_TILE;
 #pragma unroll
-    for (int w = 0; w < OS_WAVES; ++w) {
+    for (int w = 0; w < OS_WAVES; ++w)
         s_wh[w][t] = 0;
-        s_match[w][t] = 0;
-    }
     const bool dv = t < nbins;
     const uint32_t tot = dv ? totals[t] : 0u;
     const uint32_t hcur = dv ? hist[(size_t)t * ntiles + tile] : 0u;
@@ -211,6 +209,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     __syncthreads();
 
     const uint64_t lt = (1ull << lane) - 1ull;
+    const int dbits = __popc(dmask);
     uint64_t item[OS_IPL];
     uint32_t dr[OS_IPL]; // digit | wave-local position << 8 | valid << 31
 #pragma unroll
@@ -224,15 +223,24 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
         else
             item[r] = valid ? reinterpret_cast<const uint64_t *>(in)[i] : 0ull;
         const uint32_t d = (uint32_t)(it_cell(item[r], pk) >> shift) & dmask;
-        if (valid)
-            atomicOr(&s_match[wv][d], (unsigned long long)(1ull << lane));
-        const uint64_t m = s_match[wv][d];
-        const uint32_t rank = (uint32_t)__popcll(m & lt);
-        const uint32_t run = s_wh[wv][d];
-        if (valid && rank == 0) {
-            s_wh[wv][d] = run + (uint32_t)__popcll(m);
-            s_match[wv][d] = 0;
+        // lanes sharing my digit: one ballot per digit bit
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            if (bit < dbits) {
+                const bool on = (d >> bit) & 1u;
+                const uint64_t bb = __ballot(on);
+                m &= on ? bb : ~bb;
+            }
         }
+        const uint32_t rank = (uint32_t)__popcll(m & lt);
+        const int leader = valid ? (int)__builtin_ctzll(m) : 0;
+        // the group's lowest lane advances the wave's running count of the
+        // digit (LDS atomics of one wave complete in issue order)
+        uint32_t run = 0;
+        if (valid && rank == 0)
+            run = atomicAdd(&s_wh[wv][d], (uint32_t)__popcll(m));
+        run = (uint32_t)__shfl((int)run, leader, 64);
         dr[r] = d | ((run + rank) << 8) | (valid ? 0x80000000u : 0u);
     }
     __syncthreads();
