@@ -104,20 +104,58 @@ __device__ static inline uint32_t find_sid(const uint64_t *__restrict__ offs, ui
 // First pass: hash key number i of the chunk (batch position base + i).  A
 // single sketch (MULTI false) hashes with d0, passed by value; a batch finds
 // the owning sketch by binary search over the batch offsets.
+// The sketches a tile of a batch spans: their offsets offs[sid0 .. sid0 + cnt]
+// copied to LDS by the block (cnt = 0: too many, search the global array).
+#define SIDW 64u
+struct sid_window {
+    const uint64_t *o;
+    uint32_t sid0, cnt;
+};
+
+__device__ static inline void load_sid_window(const uint64_t *__restrict__ offs, uint32_t nsk, uint64_t lo,
+                                              uint64_t hi, uint64_t *s_off, uint32_t *s_w)
+{
+    if (threadIdx.x == 0) {
+        const uint32_t a = find_sid(offs, nsk, lo), b = find_sid(offs, nsk, hi);
+        s_w[0] = a;
+        s_w[1] = (b - a + 1u) <= SIDW ? (b - a + 1u) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; s_w[1] && j <= s_w[1]; j += blockDim.x)
+        s_off[j] = offs[s_w[0] + j];
+    __syncthreads();
+}
+
 template <int KB, bool MULTI>
 __device__ static inline uint64_t make_item(const KeyT<KB> *__restrict__ keys, uint32_t i, uint64_t base,
                                             const qs_sdesc &d0, const qs_sdesc *__restrict__ D,
                                             const uint64_t *__restrict__ offs, uint32_t nsk,
-                                            const qs_pack &pk)
+                                            const qs_pack &pk, const sid_window &win)
 {
     const KeyT<KB> key = keys[i];
     if (!MULTI)
         return (uint64_t)qs_bucket_d(key, d0) | ((uint64_t)qs_maskidx_d(key, d0) << pk.mshift) |
                ((uint64_t)i << pk.ishift);
     const uint64_t pos = base + i;
-    const uint32_t sid = find_sid(offs, nsk, pos);
+    uint32_t sid;
+    uint64_t sstart;
+    if (win.cnt) { // win.o[lo] <= pos < win.o[hi]
+        uint32_t lo = 0, hi = win.cnt;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (win.o[mid] <= pos)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        sid = win.sid0 + lo;
+        sstart = win.o[lo];
+    } else {
+        sid = find_sid(offs, nsk, pos);
+        sstart = offs[sid];
+    }
     const qs_sdesc &d = D[sid];
-    const uint64_t start = offs[sid] > base ? offs[sid] : base; // this sketch's part of the chunk
+    const uint64_t start = sstart > base ? sstart : base; // this sketch's part of the chunk
     return (uint64_t)qs_bucket_d(key, d) | ((uint64_t)sid << pk.bb) |
            ((uint64_t)qs_maskidx_d(key, d) << pk.mshift) | ((pos - start) << pk.ishift);
 }
@@ -131,10 +169,19 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
                                                       uint32_t *__restrict__ hist)
 {
     __shared__ uint32_t s_h[OS_BINS];
+    __shared__ uint64_t s_off[MULTI ? SIDW + 1 : 1];
+    __shared__ uint32_t s_w[2];
     const uint32_t t = threadIdx.x;
     s_h[t] = 0;
-    __syncthreads();
     const uint32_t tb = blockIdx.x * OS_TILE;
+    sid_window win{s_off, 0u, 0u};
+    if (FIRST && MULTI) {
+        const uint32_t te = (n - tb) < OS_TILE ? n : tb + OS_TILE;
+        load_sid_window(offs, nsk, base + tb, base + te - 1u, s_off, s_w);
+        win.sid0 = s_w[0];
+        win.cnt = s_w[1];
+    }
+    __syncthreads();
 #pragma unroll 4
     for (int r = 0; r < OS_IPL; ++r) {
         const uint32_t i = tb + r * OS_THREADS + t;
@@ -142,7 +189,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
             uint64_t c;
             if (FIRST)
                 c = it_cell(make_item<KB, MULTI>(reinterpret_cast<const KeyT<KB> *>(in), i, base, d0, D, offs,
-                                                 nsk, pk),
+                                                 nsk, pk, win),
                             pk);
             else
                 c = it_cell(reinterpret_cast<const uint64_t *>(in)[i], pk);
@@ -192,9 +239,18 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     __shared__ uint32_t s_ls[OS_BINS];
     __shared__ uint32_t s_goff[OS_BINS];
     __shared__ uint32_t s_red[OS_WAVES];
+    __shared__ uint64_t s_off[MULTI ? SIDW + 1 : 1];
+    __shared__ uint32_t s_w[2];
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
     const uint32_t tile = blockIdx.x;
     const uint32_t tb = tile * OS_TILE;
+    sid_window win{s_off, 0u, 0u};
+    if (FIRST && MULTI) {
+        const uint32_t te = (n - tb) < OS_TILE ? n : tb + OS_TILE;
+        load_sid_window(offs, nsk, base + tb, base + te - 1u, s_off, s_w);
+        win.sid0 = s_w[0];
+        win.cnt = s_w[1];
+    }
 #pragma unroll
     for (int w = 0; w < OS_WAVES; ++w)
         s_wh[w][t] = 0;
@@ -218,7 +274,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
         const bool valid = i < n;
         if (FIRST)
             item[r] = valid ? make_item<KB, MULTI>(reinterpret_cast<const KeyT<KB> *>(in), i, base, d0, D, offs,
-                                                   nsk, pk)
+                                                   nsk, pk, win)
                             : 0ull;
         else
             item[r] = valid ? reinterpret_cast<const uint64_t *>(in)[i] : 0ull;
@@ -299,9 +355,13 @@ __device__ static inline seg_tables tables_of(uint32_t sid, const qs_sdesc *__re
 
 template <bool MULTI>
 __device__ static inline uint32_t mask_of(uint32_t sid, uint32_t mu, const qs_sdesc *__restrict__ D,
-                                          const uint32_t *__restrict__ mask_pool, const uint32_t *s_masks)
+                                          const uint32_t *__restrict__ mask_pool, const uint32_t *s_masks,
+                                          uint32_t unm)
 {
-    return MULTI ? mask_pool[D[sid].masks_off + mu] : s_masks[mu];
+    if (!MULTI)
+        return s_masks[mu];
+    // equal-sized mask tables: the offset needs no descriptor load
+    return mask_pool[(unm ? sid * unm : D[sid].masks_off) + mu];
 }
 
 // packed per-byte increments of one 3-bit mask into 7 words
@@ -443,7 +503,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
         const uint32_t b = it_b(it, pk);
         uint32_t M = 0;
         if (valid)
-            M = mask_of<MULTI>(sid, it_mu(it, pk), D, mask_pool, s_masks);
+            M = mask_of<MULTI>(sid, it_mu(it, pk), D, mask_pool, s_masks, pk.unm);
         uint64_t cr = shfl64(c, lane == 63 ? 63 : (int)lane + 1);
         if (lane == 63)
             cr = (p + 1u < n) ? it_cell(IT[p + 1u], pk) : END_CELL;
@@ -493,7 +553,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
                 const uint64_t iq = IT[q];
                 in = it_cell(iq, pk) == c63;
                 if (in)
-                    Mq = mask_of<MULTI>(sid63, it_mu(iq, pk), D, mask_pool, s_masks);
+                    Mq = mask_of<MULTI>(sid63, it_mu(iq, pk), D, mask_pool, s_masks, pk.unm);
             }
             const uint64_t im = __ballot(in);
             const uint32_t head_pos = (uint32_t)w0 + s63;
@@ -602,7 +662,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             const bool v = p < n;
             const uint64_t it = v ? IT[p] : 0ull;
             c[k] = v ? it_cell(it, pk) : NO_CELL;
-            M[k] = v ? mask_of<MULTI>(it_sid(it, pk), it_mu(it, pk), D, mask_pool, s_masks) : 0u;
+            M[k] = v ? mask_of<MULTI>(it_sid(it, pk), it_mu(it, pk), D, mask_pool, s_masks, pk.unm) : 0u;
         }
         // cell before my first item and after my last one
         uint64_t cprev = shfl64w<W>(c[IPL - 1], (int)gl - 1);
@@ -731,7 +791,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
                 if (q < n) {
                     const uint64_t iq = IT[q];
                     if (it_cell(iq, pk) == ce) {
-                        mask_to_words(mask_of<MULTI>(sid, it_mu(iq, pk), D, mask_pool, s_masks), tmp);
+                        mask_to_words(mask_of<MULTI>(sid, it_mu(iq, pk), D, mask_pool, s_masks, pk.unm), tmp);
                         add_words(sum, tmp);
                         ++cnt;
                     }
@@ -1136,7 +1196,7 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
                 }
                 if (act) {
                     const uint32_t mu = it_mu(it, pk);
-                    tail_add(rows, lane, mask_of<MULTI>((uint32_t)(c >> pk.bb), mu, D, mask_pool, s_masks));
+                    tail_add(rows, lane, mask_of<MULTI>((uint32_t)(c >> pk.bb), mu, D, mask_pool, s_masks, pk.unm));
                 }
                 continue;
             }
@@ -1157,7 +1217,7 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
                 }
                 if (in) {
                     const uint32_t mu = it_mu(it, pk);
-                    tail_add(rows, lane, mask_of<MULTI>((uint32_t)(c >> pk.bb), mu, D, mask_pool, s_masks));
+                    tail_add(rows, lane, mask_of<MULTI>((uint32_t)(c >> pk.bb), mu, D, mask_pool, s_masks, pk.unm));
                 }
                 am &= ~__ballot(in);
             }

@@ -123,7 +123,8 @@ struct qs_pack {
     uint32_t mshift;   // bb + sb
     uint32_t ishift;   // bb + sb + bm
     uint32_t mmask;    // (1 << bm) - 1
-    uint32_t pad;
+    uint32_t unm;      // every sketch's mask table size when they are all equal (else 0):
+                       // sketch sid's masks then start at sid * unm in the mask pool
     uint64_t bmask;    // (1 << bb) - 1
     uint64_t cmask;    // (1 << (bb + sb)) - 1: the cell (sketch, bucket)
 };
@@ -154,7 +155,7 @@ static inline bool qs_make_pack(uint64_t max_low, uint32_t nsk, uint64_t max_mas
     p.mshift = p.bb + p.sb;
     p.ishift = p.bb + p.sb + p.bm;
     p.mmask = (1u << p.bm) - 1u;
-    p.pad = 0;
+    p.unm = 0;
     p.bmask = (1ull << p.bb) - 1ull;
     p.cmask = (1ull << (p.bb + p.sb)) - 1ull;
     *pk = p;

@@ -380,6 +380,10 @@ static int create_set(const qs_params *ps, uint32_t nsk, qs_sketch **out)
     qs_pack pk;
     if (!qs_make_pack(max_low, nsk, max_masks, &pk))
         return QS_ERR_PARAM; // too many sketches / too large tables for one set
+    pk.unm = ps[0].n_masks;
+    for (uint32_t i = 1; i < nsk; ++i)
+        if (ps[i].n_masks != ps[0].n_masks)
+            pk.unm = 0;
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
