@@ -96,9 +96,15 @@ __device__ static inline uint32_t find_sid(const uint64_t *__restrict__ offs, ui
 // ===========================================================================
 #define OS_THREADS 256
 #define OS_WAVES 4
-#define OS_IPL 16                         // items per lane
-#define OS_WAVE_ITEMS (64 * OS_IPL)       // 1024 items per wave
-#define OS_TILE (OS_THREADS * OS_IPL)     // 4096 items per tile
+// items per lane: 8 measured best on MI355X (2^28-key Zipf insert: 16 items
+// per lane left the scatter at 4 waves/SIMD, 4 grew the scan of the tile
+// histograms more than the scatter gained)
+#ifndef QS_OS_IPL
+#define QS_OS_IPL 8
+#endif
+#define OS_IPL QS_OS_IPL
+#define OS_WAVE_ITEMS (64 * OS_IPL)       // items per wave
+#define OS_TILE (OS_THREADS * OS_IPL)     // items per tile
 #define OS_BINS 256                       // LDS capacity for digits up to 8 bits
 
 // First pass: hash key number i of the chunk (batch position base + i).  A
@@ -223,8 +229,8 @@ __device__ static inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_red
     return off + x - v;
 }
 
-// Stable scatter of one tile.  Wave w ranks its 1024 consecutive items in 16
-// rounds: the ballot match gives the lanes sharing a digit, the lowest of them
+// Stable scatter of one tile.  Wave w ranks its OS_WAVE_ITEMS consecutive
+// items in OS_IPL rounds: the ballot match gives the lanes sharing a digit, the lowest of them
 // advances the wave's running count.  The tile is then staged digit-sorted in
 // LDS and written out as contiguous digit runs.
 template <int KB, bool FIRST, bool MULTI>
