@@ -219,8 +219,8 @@ void qs_graph_destroy(qs_graph *g);
 /* ---- insert shape and autotuner (build addition) --------------------------
  * The per-bucket-segment pass of the insert runs in groups of `width` lanes
  * (a sub-wavefront: 8, 16, 32 or 64) over `items_per_lane` (1, 2, 4 or 8)
- * consecutive sorted inserts per lane; (0, 0) selects the default 64-lane,
- * one-item kernel.  The shape never changes the result.  The autotuner runs
+ * consecutive sorted inserts per lane; (0, 0) selects the original 64-lane,
+ * one-item kernel.  A new sketch or set starts with (64, 4).  The shape never changes the result.  The autotuner runs
  * one insert batch with all 16 shapes on a twin set (the caller's set is not
  * modified; the twin needs as much device memory as the set), checks that all
  * of them leave the same state, and keeps the shape with the fastest segment

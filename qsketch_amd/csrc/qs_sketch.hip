@@ -394,6 +394,10 @@ static int create_set(const qs_params *ps, uint32_t nsk, qs_sketch **out)
     s->nsk = nsk;
     s->key_bytes = ps[0].key_bytes;
     s->pk = pk;
+    // segment-pass shape: 64-lane groups, 4 items per lane (the autotuner's
+    // pick for both the 2^28-key Zipf stream and the 4096-sketch batch)
+    s->seg_w = 64;
+    s->seg_ipl = 4;
     s->max_stream = 1ull << pk.ib;
     if (hipGetDevice(&s->device) != hipSuccess) {
         free(s);

@@ -51,10 +51,14 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const uint32_t *__re
 
 // Block d scans hist[d][0..ntiles) in place (exclusive) and writes the digit
 // total to totals[d].
+#define SCAN_PER_THREAD 8u
+
 __global__ __launch_bounds__(1024) void k_sort_scan_digit(uint32_t *__restrict__ hist,
                                                           uint32_t ntiles,
                                                           uint32_t *__restrict__ totals)
 {
+    // each thread scans SCAN_PER_THREAD consecutive tiles, the block scans the
+    // thread sums: 8192 tiles per iteration
     __shared__ uint32_t s_w[16];
     __shared__ uint32_t s_carry;
     const uint32_t t = threadIdx.x;
@@ -63,11 +67,17 @@ __global__ __launch_bounds__(1024) void k_sort_scan_digit(uint32_t *__restrict__
     if (t == 0)
         s_carry = 0;
     __syncthreads();
-    for (uint32_t base = 0; base < ntiles; base += 1024) {
-        uint32_t i = base + t;
-        uint32_t v = i < ntiles ? row[i] : 0u;
-        // inclusive wave scan
-        uint32_t x = v;
+    for (uint32_t base = 0; base < ntiles; base += 1024u * SCAN_PER_THREAD) {
+        const uint32_t i0 = base + t * SCAN_PER_THREAD;
+        uint32_t v[SCAN_PER_THREAD];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+            v[k] = (i0 + k < ntiles) ? row[i0 + k] : 0u;
+            sum += v[k];
+        }
+        // inclusive wave scan of the thread sums
+        uint32_t x = sum;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             uint32_t y = __shfl_up(x, o, 64);
@@ -89,9 +99,14 @@ __global__ __launch_bounds__(1024) void k_sort_scan_digit(uint32_t *__restrict__
             s_w[t] = sc - a; // exclusive wave offsets
         }
         __syncthreads();
-        uint32_t carry = s_carry;
-        if (i < ntiles)
-            row[i] = carry + s_w[wv] + x - v;
+        const uint32_t carry = s_carry;
+        uint32_t run = carry + s_w[wv] + x - sum; // exclusive prefix of my first tile
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+            if (i0 + k < ntiles)
+                row[i0 + k] = run;
+            run += v[k];
+        }
         __syncthreads();
         if (t == 1023)
             s_carry = carry + s_w[wv] + x;
