@@ -328,7 +328,7 @@ class Sketch_GPU_Sub_Warp_Level:
         check(_lib().qs_sync(self._h), "qs_sync")
 
     def set_shape(self, width: int, items_per_lane: int) -> None:
-        """Segment-pass shape: sub-wavefront width x items per lane ((0, 0) = default)."""
+        """Segment-pass shape: sub-wavefront width x items per lane (new sketches: (64, 4); (0, 0) = the 64-lane one-item kernel)."""
         check(_lib().qs_set_insert_shape(self._h, int(width), int(items_per_lane)), "qs_set_insert_shape")
 
     def shape(self):
@@ -481,7 +481,7 @@ class SketchSet:
         check(_lib().qs_sync(self._h), "qs_sync")
 
     def set_shape(self, width: int, items_per_lane: int) -> None:
-        """Segment-pass shape: sub-wavefront width x items per lane ((0, 0) = default)."""
+        """Segment-pass shape: sub-wavefront width x items per lane (new sketches: (64, 4); (0, 0) = the 64-lane one-item kernel)."""
         check(_lib().qs_set_insert_shape(self._h, int(width), int(items_per_lane)), "qs_set_insert_shape")
 
     def shape(self):
