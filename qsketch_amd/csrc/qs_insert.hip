@@ -204,7 +204,92 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
     }
     __syncthreads();
     if (t < nbins)
-        hist[(size_t)t * ntiles + blockIdx.x] = s_h[t];
+        hist[(size_t)blockIdx.x * nbins + t] = s_h[t]; // tile-major: one contiguous row per tile
+}
+
+// Two-level exclusive scan over tiles of the tile-major digit counts.
+// Level 1: each block takes SCAN_TPB consecutive tiles, thread d = digit d
+// (reads and writes are contiguous rows), and leaves group-local prefixes plus
+// the group's sums.  Level 2: one block per digit scans that digit's group
+// sums and writes its total.  The scatter adds the two levels.
+#define SCAN_TPB 16u
+
+__global__ __launch_bounds__(256) void k_hist_scan_local(uint32_t *__restrict__ hist, uint32_t ntiles,
+                                                         uint32_t nbins, uint32_t *__restrict__ gsum)
+{
+    const uint32_t d = threadIdx.x;
+    if (d >= nbins)
+        return;
+    const uint32_t t0 = blockIdx.x * SCAN_TPB;
+    const uint32_t t1 = min(t0 + SCAN_TPB, ntiles);
+    uint32_t run = 0;
+#pragma unroll 8
+    for (uint32_t t = t0; t < t1; ++t) {
+        uint32_t *h = hist + (size_t)t * nbins + d;
+        const uint32_t v = *h;
+        *h = run;
+        run += v;
+    }
+    gsum[(size_t)blockIdx.x * nbins + d] = run;
+}
+
+// one block per digit: a block-wide scan of the group sums (8 per thread)
+#define GSCAN_PT 8u
+__global__ __launch_bounds__(1024) void k_hist_scan_groups(uint32_t *__restrict__ gsum, uint32_t ngrp,
+                                                           uint32_t nbins, uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_carry;
+    const uint32_t d = blockIdx.x, t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    if (t == 0)
+        s_carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < ngrp; base += 1024u * GSCAN_PT) {
+        const uint32_t g0 = base + t * GSCAN_PT;
+        uint32_t v[GSCAN_PT];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < GSCAN_PT; ++k) {
+            v[k] = (g0 + k < ngrp) ? gsum[(size_t)(g0 + k) * nbins + d] : 0u;
+            sum += v[k];
+        }
+        uint32_t x = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o)
+                x += y;
+        }
+        if (lane == 63)
+            s_w[wv] = x;
+        __syncthreads();
+        if (t < 16) {
+            const uint32_t a0 = s_w[t];
+            uint32_t sc = a0;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const uint32_t y = __shfl_up(sc, o, 16);
+                if ((t & 15u) >= (uint32_t)o)
+                    sc += y;
+            }
+            s_w[t] = sc - a0;
+        }
+        __syncthreads();
+        const uint32_t carry = s_carry;
+        uint32_t run = carry + s_w[wv] + x - sum;
+#pragma unroll
+        for (uint32_t k = 0; k < GSCAN_PT; ++k) {
+            if (g0 + k < ngrp)
+                gsum[(size_t)(g0 + k) * nbins + d] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (t == 1023)
+            s_carry = carry + s_w[wv] + x;
+        __syncthreads();
+    }
+    if (t == 0)
+        totals[d] = s_carry;
 }
 
 // exclusive block scan of one value per thread (256 threads)
@@ -237,7 +322,7 @@ template <int KB, bool FIRST, bool MULTI>
 __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const void *__restrict__ in, uint32_t n, uint64_t base, qs_sdesc d0, const qs_sdesc *__restrict__ D,
     const uint64_t *__restrict__ offs, uint32_t nsk, qs_pack pk, int shift, uint32_t dmask,
-    uint32_t nbins, uint32_t ntiles, const uint32_t *__restrict__ hist,
+    uint32_t nbins, uint32_t ntiles, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ gsum,
     const uint32_t *__restrict__ totals, uint64_t *__restrict__ out)
 {
     __shared__ uint64_t s_item[OS_TILE];
@@ -262,13 +347,9 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
         s_wh[w][t] = 0;
     const bool dv = t < nbins;
     const uint32_t tot = dv ? totals[t] : 0u;
-    const uint32_t hcur = dv ? hist[(size_t)t * ntiles + tile] : 0u;
-    const uint32_t hnxt = dv ? ((tile + 1 < ntiles) ? hist[(size_t)t * ntiles + tile + 1] : tot) : 0u;
+    // items of digit t before this tile (tile-major counts, two-level scan)
+    const uint32_t hcur = dv ? hist[(size_t)tile * nbins + t] + gsum[(size_t)(tile / SCAN_TPB) * nbins + t] : 0u;
     const uint32_t dbase = block_excl_scan256(tot, s_red);
-    const uint32_t lstart = block_excl_scan256(hnxt - hcur, s_red);
-    s_ls[t] = lstart;
-    s_goff[t] = dbase + hcur - lstart;
-    __syncthreads();
 
     const uint64_t lt = (1ull << lane) - 1ull;
     const int dbits = __popc(dmask);
@@ -307,11 +388,16 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     }
     __syncthreads();
     {
-        uint32_t c0 = s_wh[0][t], c1 = s_wh[1][t], c2 = s_wh[2][t];
+        // this tile's count of digit t comes from the ranking itself
+        const uint32_t c0 = s_wh[0][t], c1 = s_wh[1][t], c2 = s_wh[2][t], c3 = s_wh[3][t];
+        const uint32_t cnt = c0 + c1 + c2 + c3;
         s_wh[0][t] = 0;
         s_wh[1][t] = c0;
         s_wh[2][t] = c0 + c1;
         s_wh[3][t] = c0 + c1 + c2;
+        const uint32_t lstart = block_excl_scan256(cnt, s_red);
+        s_ls[t] = lstart;
+        s_goff[t] = dbase + hcur - lstart;
     }
     __syncthreads();
 #pragma unroll
@@ -1283,7 +1369,8 @@ uint64_t qs_scratch_hist_words(uint64_t chunk)
     uint64_t tiles = (chunk + OS_TILE - 1) / OS_TILE;
     if (tiles == 0)
         tiles = 1;
-    uint64_t a = OS_BINS * tiles + OS_BINS;
+    // tile-major counts + group sums + digit totals
+    uint64_t a = OS_BINS * tiles + OS_BINS * ((tiles + SCAN_TPB - 1) / SCAN_TPB) + OS_BINS;
     uint64_t b = qs_sort_hist_words((uint32_t)(chunk < 0xffffffffull ? chunk : 0xffffffffull));
     return a > b ? a : b;
 }
@@ -1347,7 +1434,9 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
     const int passes = (bits + 7) / 8;
     const int dbits = (bits + passes - 1) / passes; // <= 8
     const uint32_t nbins = 1u << dbits, dmask = nbins - 1u;
-    uint32_t *totals = s->d_hist + (size_t)OS_BINS * ntiles;
+    const uint32_t ngrp = (ntiles + SCAN_TPB - 1) / SCAN_TPB;
+    uint32_t *gsum = s->d_hist + (size_t)OS_BINS * ntiles;
+    uint32_t *totals = gsum + (size_t)OS_BINS * ngrp;
     const qs_sdesc d0 = s->h_desc[0];
     const void *in = keys;
     int out = 0;
@@ -1368,7 +1457,8 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
             qs_tim_end(s, e0, stream);
             qs_tim_begin(s, e1, QS_TK_SORT_SCAN, cn, stream);
         }
-        qs_scan_digits(s->d_hist, ntiles, nbins, totals, stream);
+        hipLaunchKernelGGL(k_hist_scan_local, dim3(ngrp), dim3(256), 0, stream, s->d_hist, ntiles, nbins, gsum);
+        hipLaunchKernelGGL(k_hist_scan_groups, dim3(nbins), dim3(1024), 0, stream, gsum, ngrp, nbins, totals);
         if (bc.timed) {
             qs_tim_end(s, e1, stream);
             qs_tim_begin(s, e2, QS_TK_SORT_DOWN, cn, stream);
@@ -1376,11 +1466,11 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
         if (ps == 0)
             hipLaunchKernelGGL((k_bs_down<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               s->d_hist, totals, s->d_it[out]);
+                               s->d_hist, gsum, totals, s->d_it[out]);
         else
             hipLaunchKernelGGL((k_bs_down<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               s->d_hist, totals, s->d_it[out]);
+                               s->d_hist, gsum, totals, s->d_it[out]);
         if (bc.timed)
             qs_tim_end(s, e2, stream);
         in = s->d_it[out];
