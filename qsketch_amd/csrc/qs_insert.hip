@@ -1199,7 +1199,7 @@ __device__ static inline void tail_add(uint32_t (*rows)[TAIL_ROW], uint32_t lane
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         if (M) {
-            rows[lane][__builtin_ctz(M)] += 1u;
+            atomicAdd(&rows[lane][__builtin_ctz(M)], 1u); // one ds_add: the row is this lane's own
             M &= M - 1u;
         }
     }
@@ -1261,10 +1261,13 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
         uint64_t c_c = NO_CELL;
         uint32_t *c_row = nullptr;
         uint32_t c_tail = 0;
+        uint64_t nxt = (w0 + lane < wend) ? IT[w0 + lane] : 0ull;
         for (uint64_t s0 = w0; s0 < wend; s0 += 64u) {
             const uint32_t p = (uint32_t)s0 + lane;
             const bool valid = p < wend;
-            const uint64_t it = valid ? IT[p] : 0ull;
+            const uint64_t it = nxt;
+            // the next step's items are loaded while this step is processed
+            nxt = (s0 + 64u + lane < wend) ? IT[s0 + 64u + lane] : 0ull;
             const uint64_t c = valid ? it_cell(it, pk) : NO_CELL;
             const uint64_t c_first = shfl64(c, 0);
             const uint32_t last_lane = (wend - s0 < 64u) ? (uint32_t)(wend - s0 - 1u) : 63u;
