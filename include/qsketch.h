@@ -42,6 +42,7 @@ extern "C" {
 #define QS_ERR_HIP           -4  /* a HIP runtime call failed                */
 #define QS_ERR_STATE         -5  /* imported state is malformed              */
 #define QS_ERR_NODEVICE      -6  /* no usable GPU                            */
+#define QS_ERR_INTERNAL      -8  /* a device-side consistency check failed; state invalid until qs_clear */
 
 /* ---- insert semantics of the allocation-triggering insert --------------- */
 #define QS_LITERAL 0 /* v3:187 as written: trigger insert adds to g[j mod 8] */

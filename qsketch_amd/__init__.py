@@ -26,6 +26,7 @@ QS_ERR_HIP = -4
 QS_ERR_STATE = -5
 QS_ERR_NODEVICE = -6
 QS_ERR_TABLE = -7
+QS_ERR_INTERNAL = -8
 QS_LITERAL = 0
 QS_PAPER = 1
 MLP_PARAMS = 8385

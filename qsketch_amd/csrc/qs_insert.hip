@@ -42,7 +42,8 @@
 //   k_assign  allocation ids per sketch, in stream order of the triggering inserts.
 //   k_tail    long segments after their allocation point: per-lane LDS counter
 //             rows over windows of sorted items, one atomic per counter per
-//             window and bucket.
+//             window and bucket.  Only the runs at the two ends of a 64-item
+//             step can be long, so a step looks up at most two rows.
 #include "qs_internal.h"
 
 #include <cstring>
@@ -188,10 +189,13 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
         win.cnt = s_w[1];
     }
     __syncthreads();
+    const uint32_t lane = t & 63u;
 #pragma unroll 4
     for (int r = 0; r < OS_IPL; ++r) {
-        const uint32_t i = tb + r * OS_THREADS + t;
-        if (i < n) {
+        const uint32_t i = tb + r * OS_THREADS + t; // a wavefront's lanes hold consecutive items
+        const bool valid = i < n;
+        uint32_t d = 0;
+        if (valid) {
             uint64_t c;
             if (FIRST)
                 c = it_cell(make_item<KB, MULTI>(reinterpret_cast<const KeyT<KB> *>(in), i, base, d0, D, offs,
@@ -199,8 +203,19 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
                             pk);
             else
                 c = it_cell(reinterpret_cast<const uint64_t *>(in)[i], pk);
-            atomicAdd(&s_h[(uint32_t)(c >> shift) & dmask], 1u);
+            d = (uint32_t)(c >> shift) & dmask;
         }
+        // one LDS add per run of equal digits on adjacent lanes (a Zipf-hot
+        // cell fills whole wavefronts, whose adds to one bin would serialise)
+        // (both ballots are taken by the whole wavefront, outside any branch)
+        const uint32_t dp = (uint32_t)__shfl_up((int)d, 1, 64);
+        const bool head = valid && (lane == 0 || d != dp);
+        const uint64_t heads = __ballot(head);
+        const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid)); // valid lanes are a prefix
+        const uint64_t later = heads & ~((2ull << lane) - 1ull);
+        const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : nvalid;
+        if (head)
+            atomicAdd(&s_h[d], end - lane);
     }
     __syncthreads();
     if (t < nbins)
@@ -323,7 +338,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const void *__restrict__ in, uint32_t n, uint64_t base, qs_sdesc d0, const qs_sdesc *__restrict__ D,
     const uint64_t *__restrict__ offs, uint32_t nsk, qs_pack pk, int shift, uint32_t dmask,
     uint32_t nbins, uint32_t ntiles, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ gsum,
-    const uint32_t *__restrict__ totals, uint64_t *__restrict__ out)
+    const uint32_t *__restrict__ totals, uint64_t *__restrict__ out, qs_dev_state *__restrict__ st)
 {
     __shared__ uint64_t s_item[OS_TILE];
     __shared__ uint32_t s_wh[OS_WAVES][OS_BINS];
@@ -411,7 +426,11 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const uint32_t tn = (n - tb) < OS_TILE ? (n - tb) : OS_TILE;
     for (uint32_t q = t; q < tn; q += OS_THREADS) {
         const uint64_t it = s_item[q];
-        out[s_goff[(uint32_t)(it_cell(it, pk) >> shift) & dmask] + q] = it;
+        const uint32_t dst = s_goff[(uint32_t)(it_cell(it, pk) >> shift) & dmask] + q;
+        if (dst < n) // the tile counts and the ranking agree; checked, never trusted
+            out[dst] = it;
+        else
+            atomicOr(&st->err, QS_EF_SORT);
     }
 }
 
@@ -459,20 +478,11 @@ __device__ static inline uint32_t mask_of(uint32_t sid, uint32_t mu, const qs_sd
 // packed per-byte increments of one 3-bit mask into 7 words
 __device__ static inline void mask_to_words(uint32_t M, uint32_t a[7])
 {
+    // word k holds counters 4k..4k+3: multiplying nibble n by 1 + 2^7 + 2^14 +
+    // 2^21 puts its bit i at bit 8i (the four shifted copies do not overlap)
 #pragma unroll
     for (int k = 0; k < 7; ++k)
-        a[k] = 0;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        if (M) {
-            uint32_t j = (uint32_t)__builtin_ctz(M);
-            M &= M - 1u;
-            uint32_t inc = 1u << ((j & 3u) * 8u);
-#pragma unroll
-            for (int k = 0; k < 7; ++k)
-                a[k] += ((j >> 2) == (uint32_t)k) ? inc : 0u;
-        }
-    }
+        a[k] = (((M >> (4 * k)) & 0xfu) * 0x00204081u) & 0x01010101u;
 }
 
 // Owner of a complete segment (packed per-byte counts a[], each <= 128):
@@ -963,7 +973,7 @@ __device__ static inline void flush_trig(qs_trig (*s_tr)[TBUF], uint32_t wv, uin
             if (lane < sizeof(qs_trig) / 4u)
                 dst[lane] = src[lane];
         } else if (lane == 0) {
-            atomicOr(&st->err, 1u); // more allocations than the pools hold
+            atomicOr(&st->err, QS_EF_POOL); // more allocations than the pools hold
         }
     }
     ntb = 0;
@@ -1160,7 +1170,7 @@ __global__ __launch_bounds__(256) void k_assign(const qs_trig *__restrict__ trig
         const uint32_t row = id - QS_LINK_START;
         if (row >= d.n_high) {
             if (lane == 0)
-                atomicOr(&st->err, 1u);
+                atomicOr(&st->err, QS_EF_POOL);
             continue;
         }
         if (lane < 32u)
@@ -1175,34 +1185,47 @@ __global__ __launch_bounds__(256) void k_assign(const qs_trig *__restrict__ trig
 // Long segments: every insert of a bucket linked before this chunk, and the
 // inserts after the allocation batch of a bucket linked in this chunk, are
 // commutative high adds.  A wavefront walks a window of TAIL_WINDOW sorted
-// positions; each lane counts its inserts' counters in its own LDS row while
-// the bucket stays the same, and the rows are summed and flushed with one
-// atomic per counter when it changes.
-#define TAIL_ROW 29 // 28 counters, odd stride against bank conflicts
+// positions in steps of 64 (loading TAIL_PF steps ahead); each lane counts its
+// inserts' counters as packed bytes in registers while the bucket stays the
+// same (at most one insert per step, so a byte holds at most 64 in a window),
+// and the wavefront sums the bytes and adds them with one atomic per counter
+// when the bucket changes.
+#define TAIL_PF 4
 
-__device__ static inline void tail_flush(uint32_t (*rows)[TAIL_ROW], uint32_t lane, uint32_t *row)
+// lane j < 28: the wavefront's sum of byte j of acc (byte j & 3 of word j >> 2)
+__device__ static inline uint32_t wave_byte_sum(const uint32_t acc[7], uint32_t lane)
 {
-    if (lane < 28u) {
-        uint32_t sum = 0;
-        for (uint32_t l = 0; l < 64u; ++l)
-            sum += rows[l][lane];
-        if (sum)
-            atomicAdd(row + lane, sum);
+    uint32_t lo[7], hi[7]; // bytes 0 and 2, bytes 1 and 3, as 16-bit halves
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        lo[k] = acc[k] & 0x00ff00ffu;
+        hi[k] = (acc[k] >> 8) & 0x00ff00ffu;
     }
 #pragma unroll
-    for (int j = 0; j < 28; ++j)
-        rows[lane][j] = 0;
-}
-
-__device__ static inline void tail_add(uint32_t (*rows)[TAIL_ROW], uint32_t lane, uint32_t M)
-{
+    for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        if (M) {
-            atomicAdd(&rows[lane][__builtin_ctz(M)], 1u); // one ds_add: the row is this lane's own
-            M &= M - 1u;
+        for (int k = 0; k < 7; ++k) {
+            lo[k] += (uint32_t)__shfl_xor((int)lo[k], o, 64);
+            hi[k] += (uint32_t)__shfl_xor((int)hi[k], o, 64);
         }
     }
+    const uint32_t kw = lane >> 2, by = lane & 3u;
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        v = (kw == (uint32_t)k) ? ((by & 1u) ? hi[k] : lo[k]) : v;
+    v = (by & 2u) ? (v >> 16) : (v & 0xffffu);
+    return lane < 28u ? v : 0u;
+}
+
+__device__ static inline void tail_flush(uint32_t acc[7], uint32_t lane, uint32_t *row)
+{
+    const uint32_t v = wave_byte_sum(acc, lane);
+    if (v)
+        atomicAdd(row + lane, v);
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        acc[k] = 0;
 }
 
 // high row that receives the tail inserts of cell (sid, b), or null; tail =
@@ -1237,88 +1260,169 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
                                               const uint32_t *__restrict__ longbm,
                                               const uint32_t *__restrict__ tail_pool,
                                               const uint32_t *__restrict__ first_new,
+                                              const uint32_t *__restrict__ lng, bool short_runs,
                                               const qs_dev_state *__restrict__ st)
 {
     if (st->n_long == 0)
         return;
     __shared__ uint32_t s_masks[MULTI ? 1 : QS_MAX_MASKS];
-    __shared__ uint32_t s_rows[4][64][TAIL_ROW];
     if (!MULTI)
         for (uint32_t i = threadIdx.x; i < n_masks0; i += blockDim.x)
             s_masks[i] = mask_pool[i];
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    uint32_t(*rows)[TAIL_ROW] = s_rows[wv];
-#pragma unroll
-    for (int j = 0; j < 28; ++j)
-        rows[lane][j] = 0;
     __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t gw = blockIdx.x * 4u + wv;
     const uint32_t nw = gridDim.x * 4u;
     for (uint64_t w0 = (uint64_t)gw * TAIL_WINDOW; w0 < n; w0 += (uint64_t)nw * TAIL_WINDOW) {
         const uint64_t wend = (w0 + TAIL_WINDOW < n) ? w0 + TAIL_WINDOW : n;
         uint32_t *cur = nullptr;
-        // lookup cache for the cell of the previous single-cell step
+        uint32_t acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        // lookup cache: the last cell whose row was fetched
         uint64_t c_c = NO_CELL;
         uint32_t *c_row = nullptr;
         uint32_t c_tail = 0;
-        uint64_t nxt = (w0 + lane < wend) ? IT[w0 + lane] : 0ull;
-        for (uint64_t s0 = w0; s0 < wend; s0 += 64u) {
-            const uint32_t p = (uint32_t)s0 + lane;
-            const bool valid = p < wend;
-            const uint64_t it = nxt;
-            // the next step's items are loaded while this step is processed
-            nxt = (s0 + 64u + lane < wend) ? IT[s0 + 64u + lane] : 0ull;
-            const uint64_t c = valid ? it_cell(it, pk) : NO_CELL;
-            const uint64_t c_first = shfl64(c, 0);
-            const uint32_t last_lane = (wend - s0 < 64u) ? (uint32_t)(wend - s0 - 1u) : 63u;
-            const uint64_t c_last = shfl64(c, (int)last_lane);
-            if (c_first == c_last) {
-                // the whole step is one cell (the items are sorted by cell)
-                if (c_first != c_c) {
-                    c_c = c_first;
-                    c_row = tail_row<MULTI>((uint32_t)(c_c >> pk.bb), (uint32_t)(c_c & pk.bmask), D, low_pool,
-                                            high_pool, tail_pool, longbm, first_new, c_tail);
-                }
-                if (!c_row)
-                    continue;
-                const bool act = valid && p >= c_tail;
-                if (!__ballot(act))
-                    continue;
-                if (c_row != cur) {
-                    if (cur)
-                        tail_flush(rows, lane, cur);
-                    cur = c_row;
-                }
-                if (act) {
-                    const uint32_t mu = it_mu(it, pk);
-                    tail_add(rows, lane, mask_of<MULTI>((uint32_t)(c >> pk.bb), mu, D, mask_pool, s_masks, pk.unm));
-                }
-                continue;
+        // cell of the item before the step
+        uint64_t c_prev = (w0 > 0) ? it_cell(IT[w0 - 1u], pk) : NO_CELL;
+        uint64_t buf[TAIL_PF];
+#pragma unroll
+        for (int k = 0; k < TAIL_PF; ++k) {
+            const uint64_t q = w0 + 64u * k + lane;
+            buf[k] = q < wend ? IT[q] : 0ull;
+        }
+        for (uint64_t g = w0; g < wend; g += 64u * TAIL_PF) {
+            // the next TAIL_PF steps' items are loaded while these are processed
+            uint64_t nb[TAIL_PF];
+#pragma unroll
+            for (int k = 0; k < TAIL_PF; ++k) {
+                const uint64_t q = g + 64u * (TAIL_PF + k) + lane;
+                nb[k] = q < wend ? IT[q] : 0ull;
             }
-            uint32_t tail = 0;
-            uint32_t *row = valid ? tail_row<MULTI>((uint32_t)(c >> pk.bb), (uint32_t)(c & pk.bmask), D,
-                                                    low_pool, high_pool, tail_pool, longbm, first_new, tail)
-                                  : nullptr;
-            const bool act = row != nullptr && p >= tail;
-            uint64_t am = __ballot(act);
-            while (am) {
-                const uint32_t lead = (uint32_t)__builtin_ctzll(am);
-                const uint64_t rl = shfl64((uint64_t)(uintptr_t)row, (int)lead);
-                const bool in = act && (uint64_t)(uintptr_t)row == rl;
-                if ((uint32_t *)(uintptr_t)rl != cur) {
-                    if (cur)
-                        tail_flush(rows, lane, cur);
-                    cur = (uint32_t *)(uintptr_t)rl;
+#pragma unroll
+            for (int k = 0; k < TAIL_PF; ++k) {
+                const uint64_t s0 = g + 64u * k;
+                if (s0 >= wend)
+                    break;
+                const uint32_t p = (uint32_t)s0 + lane;
+                const bool valid = p < wend;
+                const uint64_t it = buf[k];
+                const uint64_t nx = (k + 1 < TAIL_PF) ? buf[k + 1] : nb[0]; // the next step's items
+                const uint64_t c = valid ? it_cell(it, pk) : NO_CELL;
+                const uint32_t last_lane = (wend - s0 < 64u) ? (uint32_t)(wend - s0 - 1u) : 63u;
+                const uint64_t c_first = shfl64(c, 0);
+                const uint64_t c_last = shfl64(c, (int)last_lane);
+                const uint64_t nx0 = shfl64(nx, 0); // taken by the whole wavefront
+                const uint64_t c_after = (s0 + 64u < wend) ? it_cell(nx0, pk)
+                                         : (wend < n)      ? it_cell(IT[wend], pk)
+                                                           : END_CELL;
+                // This pass takes the runs that cross a step boundary (hold two
+                // positions 64k - 1 and 64k); in every step such a run touches it
+                // holds the first or the last item.  Runs inside one step are
+                // left to the list pass below.
+                const bool cross_f = c_first == c_prev || (c_first == c_last && c_first == c_after);
+                const bool cross_l = c_last != c_first && c_last == c_after;
+                c_prev = c_last;
+                uint32_t *row_f = nullptr, *row_l = nullptr;
+                uint32_t tail_f = 0, tail_l = 0;
+                if (cross_f) {
+                    if (c_first != c_c) {
+                        c_c = c_first;
+                        c_row = tail_row<MULTI>((uint32_t)(c_c >> pk.bb), (uint32_t)(c_c & pk.bmask), D, low_pool,
+                                                high_pool, tail_pool, longbm, first_new, c_tail);
+                    }
+                    row_f = c_row;
+                    tail_f = c_tail;
                 }
-                if (in) {
-                    const uint32_t mu = it_mu(it, pk);
-                    tail_add(rows, lane, mask_of<MULTI>((uint32_t)(c >> pk.bb), mu, D, mask_pool, s_masks, pk.unm));
+                if (cross_l) {
+                    if (c_last != c_c) {
+                        // Unless the segment pass marks runs under 65 items long,
+                        // a run of at most 64 items has no tail inserts (the
+                        // replay's tails start 64 items after the head): its
+                        // length is its items here plus the next step's leading
+                        // ones, when the run ends there.
+                        bool need = true;
+                        if (!short_runs && s0 + 64u < wend) {
+                            const bool nv = s0 + 64u + lane < wend;
+                            const uint64_t nm = __ballot(nv && it_cell(nx, pk) == c_last);
+                            const uint32_t b = (~nm) ? (uint32_t)__builtin_ctzll(~nm) : 64u;
+                            const uint32_t a = (uint32_t)__popcll(__ballot(valid && c == c_last));
+                            need = a + b > 64u;
+                        }
+                        c_c = c_last;
+                        c_row = nullptr;
+                        c_tail = 0;
+                        if (need)
+                            c_row = tail_row<MULTI>((uint32_t)(c_c >> pk.bb), (uint32_t)(c_c & pk.bmask), D,
+                                                    low_pool, high_pool, tail_pool, longbm, first_new, c_tail);
+                    }
+                    row_l = c_row;
+                    tail_l = c_tail;
                 }
-                am &= ~__ballot(in);
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    uint32_t *const row = side ? row_l : row_f;
+                    if (!row)
+                        continue;
+                    const bool act = valid && c == (side ? c_last : c_first) && p >= (side ? tail_l : tail_f);
+                    if (!__ballot(act))
+                        continue;
+                    if (row != cur) {
+                        if (cur)
+                            tail_flush(acc, lane, cur);
+                        cur = row;
+                    }
+                    if (act) {
+                        uint32_t t[7];
+                        mask_to_words(mask_of<MULTI>((uint32_t)(c >> pk.bb), it_mu(it, pk), D, mask_pool, s_masks,
+                                                     pk.unm),
+                                      t);
+                        add_words(acc, t);
+                    }
+                }
             }
+#pragma unroll
+            for (int k = 0; k < TAIL_PF; ++k)
+                buf[k] = nb[k];
         }
         if (cur)
-            tail_flush(rows, lane, cur);
+            tail_flush(acc, lane, cur);
+    }
+    // List pass: a long run inside one step.  The segment pass marks runs of
+    // at least T + 1 items long (T = its window), so these exist only for
+    // windows under 64 items.  The replay also marks runs that continue after
+    // their allocation batch, but their tail starts 64 items after the head,
+    // outside a run that fits in one step.
+    if (!short_runs)
+        return;
+    const uint32_t n_long = st->n_long;
+    for (uint32_t e = gw; e < n_long; e += nw) {
+        const uint32_t h = lng[e]; // head of the run
+        const uint32_t p = h + lane;
+        const uint64_t it = p < n ? IT[p] : 0ull;
+        const uint64_t c = p < n ? it_cell(it, pk) : NO_CELL;
+        const uint64_t ch = shfl64(c, 0);
+        const uint64_t out = __ballot(c != ch);
+        const uint32_t len = out ? (uint32_t)__builtin_ctzll(out) : 64u;
+        bool cross = (h >> 6) != ((h + len - 1u) >> 6);
+        if (len == 64u && !cross)
+            cross = h + 64u < n && it_cell(IT[h + 64u], pk) == ch;
+        if (cross)
+            continue; // taken by the window pass
+        uint32_t tail = 0;
+        uint32_t *row = tail_row<MULTI>((uint32_t)(ch >> pk.bb), (uint32_t)(ch & pk.bmask), D, low_pool, high_pool,
+                                        tail_pool, longbm, first_new, tail);
+        if (!row)
+            continue;
+        const bool act = lane < len && p >= tail;
+        const uint32_t M = act ? mask_of<MULTI>((uint32_t)(ch >> pk.bb), it_mu(it, pk), D, mask_pool, s_masks, pk.unm)
+                               : 0u;
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int j = 0; j < 28; ++j) {
+            const uint32_t k = (uint32_t)__popcll(__ballot((M >> j) & 1u));
+            cnt = lane == (uint32_t)j ? k : cnt;
+        }
+        if (lane < 28u && cnt)
+            atomicAdd(row + lane, cnt);
     }
 }
 
@@ -1469,11 +1573,11 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
         if (ps == 0)
             hipLaunchKernelGGL((k_bs_down<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               s->d_hist, gsum, totals, s->d_it[out]);
+                               s->d_hist, gsum, totals, s->d_it[out], s->d_st);
         else
             hipLaunchKernelGGL((k_bs_down<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               s->d_hist, gsum, totals, s->d_it[out]);
+                               s->d_hist, gsum, totals, s->d_it[out], s->d_st);
         if (bc.timed)
             qs_tim_end(s, e2, stream);
         in = s->d_it[out];
@@ -1568,10 +1672,12 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
         qs_tim_end(s, ea, stream);
         qs_tim_begin(s, et, QS_TK_TAIL, cn, stream);
     }
+    // k_segw windows under 64 items mark runs long that fit inside one step
+    const bool short_runs = s->seg_w != 0 && s->seg_w * s->seg_ipl < 64u;
     hipLaunchKernelGGL(k_tail<MULTI>,
                        dim3(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8)), dim3(256), 0,
                        stream, IT, cn, s->d_desc, nm0, s->d_low, s->d_high, s->d_masks, s->pk, s->d_longbm,
-                       s->d_tail_of, s->d_next, s->d_st);
+                       s->d_tail_of, s->d_next, s->d_long, short_runs, s->d_st);
     hipLaunchKernelGGL(k_clear_long, dim3(grid_cap(((uint64_t)cn / 64 + 255) / 256, 4)), dim3(256), 0, stream,
                        s->d_long, IT, s->d_desc, s->pk, s->d_longbm, s->d_st);
     hipLaunchKernelGGL(k_next, dim3(grid_cap(((uint64_t)s->trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
@@ -1652,8 +1758,8 @@ static int insert_impl(qs_sketch *s, const void *d_keys, const uint64_t *offs, c
         CK(hipMemcpyAsync(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost, stream));
         CK(hipStreamSynchronize(stream));
         if (s->h_st->err) {
-            s->sticky_err = QS_ERR_POOL;
-            return QS_ERR_POOL;
+            s->sticky_err = qs_err_code(s->h_st->err);
+            return s->sticky_err;
         }
     }
     return QS_OK;

@@ -184,11 +184,20 @@ __host__ __device__ static inline uint32_t it_idx(uint64_t it, const qs_pack &pk
 }
 
 // Device-resident control block shared by a set.
+// sticky device error bits
+#define QS_EF_POOL 1u // a high pool is exhausted
+#define QS_EF_SORT 2u // a radix scatter computed a position outside the chunk
+
+static inline int qs_err_code(uint32_t err)
+{
+    return (err & QS_EF_SORT) ? QS_ERR_INTERNAL : err ? QS_ERR_POOL : QS_OK;
+}
+
 struct qs_dev_state {
     uint32_t n_ord;     // bucket segments queued for the in-order replay
     uint32_t n_long;    // long bucket segments (handled by the windowed tail kernel)
     uint32_t n_trig;    // allocations triggered in the current chunk
-    uint32_t err;       // sticky error: 1 = a high pool is exhausted
+    uint32_t err;       // sticky error bits (QS_EF_*)
     // running totals since create / clear (statistics only)
     unsigned long long replay_keys;   // inserts replayed in stream order
     unsigned long long ord_segments;  // segments sent to the replay

@@ -145,6 +145,7 @@ extern "C" const char *qs_strerror(int code)
     case QS_ERR_STATE: return "malformed state";
     case QS_ERR_NODEVICE: return "no usable GPU";
     case -7: return "exact table full (learned form: more routed keys than slots)";
+    case QS_ERR_INTERNAL: return "device consistency check failed (state invalid until qs_clear)";
     default: return "unknown error";
     }
 }
@@ -687,7 +688,7 @@ extern "C" int qs_sync(qs_sketch *s)
     CK(hipDeviceSynchronize());
     CK(hipMemcpy(s->h_st, s->d_st, sizeof(qs_dev_state), hipMemcpyDeviceToHost));
     if (s->h_st->err)
-        s->sticky_err = QS_ERR_POOL;
+        s->sticky_err = qs_err_code(s->h_st->err);
     return s->sticky_err;
 }
 
@@ -1038,6 +1039,6 @@ extern "C" int qs_get_stats(qs_sketch *s, qs_stats *st)
     st->next_id = nx[0];
     st->seed = s->h_desc[0].seed;
     st->n_sketches = s->nsk;
-    st->pool_error = h.err ? 1u : 0u;
+    st->pool_error = (h.err & QS_EF_POOL) ? 1u : 0u;
     return QS_OK;
 }
