@@ -348,7 +348,11 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     __shared__ uint64_t s_off[MULTI ? SIDW + 1 : 1];
     __shared__ uint32_t s_w[2];
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
-    const uint32_t tile = blockIdx.x;
+    // Blocks go to the 8 XCDs in turn; give each XCD a contiguous range of
+    // tiles, so neighbouring tiles' digit runs (adjacent in the output) meet
+    // in one L2 (a bijection of [0, ntiles) whatever the placement)
+    const uint32_t xq = ntiles >> 3, xr = ntiles & 7u, xb = blockIdx.x & 7u;
+    const uint32_t tile = xb * xq + min(xb, xr) + (blockIdx.x >> 3);
     const uint32_t tb = tile * OS_TILE;
     sid_window win{s_off, 0u, 0u};
     if (FIRST && MULTI) {
@@ -955,16 +959,11 @@ __device__ static inline uint32_t kth_set_bit(uint64_t m, uint32_t k)
 // chunk that links the hot buckets).
 #define TBUF 8u
 
-__device__ static inline void flush_trig(qs_trig (*s_tr)[TBUF], uint32_t wv, uint32_t lane, uint32_t &ntb,
-                                         qs_trig *__restrict__ trig, uint32_t trig_cap,
+// copy a wave's ntb staged allocations to list slots base, base + 1, ...
+__device__ static inline void write_trig(qs_trig (*s_tr)[TBUF], uint32_t wv, uint32_t lane, uint32_t ntb,
+                                         uint32_t base, qs_trig *__restrict__ trig, uint32_t trig_cap,
                                          qs_dev_state *__restrict__ st)
 {
-    if (!ntb)
-        return;
-    uint32_t base = 0;
-    if (lane == 0)
-        base = atomicAdd(&st->n_trig, ntb);
-    base = __shfl(base, 0, 64);
     for (uint32_t e = 0; e < ntb; ++e) {
         const uint32_t slot = base + e;
         if (slot < trig_cap) {
@@ -976,6 +975,19 @@ __device__ static inline void flush_trig(qs_trig (*s_tr)[TBUF], uint32_t wv, uin
             atomicOr(&st->err, QS_EF_POOL); // more allocations than the pools hold
         }
     }
+}
+
+__device__ static inline void flush_trig(qs_trig (*s_tr)[TBUF], uint32_t wv, uint32_t lane, uint32_t &ntb,
+                                         qs_trig *__restrict__ trig, uint32_t trig_cap,
+                                         qs_dev_state *__restrict__ st)
+{
+    if (!ntb)
+        return;
+    uint32_t base = 0;
+    if (lane == 0)
+        base = atomicAdd(&st->n_trig, ntb);
+    base = __shfl(base, 0, 64);
+    write_trig(s_tr, wv, lane, ntb, base, trig, trig_cap, st);
     ntb = 0;
 }
 
@@ -1113,10 +1125,35 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
                 flush_long(s_ln, wv, lane, nlb, lng, st);
         }
     }
-    flush_trig(s_tr, wv, lane, ntb, trig, trig_cap, st);
-    flush_long(s_ln, wv, lane, nlb, lng, st);
-    if (lane == 0 && replayed_all)
-        atomicAdd(&st->replay_keys, replayed_all);
+    // Final flush, one reservation per list per block: same-address atomics
+    // serialise (~11 ns each), and one per wavefront and list costs more than
+    // the replay itself.
+    __shared__ uint32_t s_nt[4], s_nl[4], s_base[2];
+    __shared__ unsigned long long s_rep[4];
+    if (lane == 0) {
+        s_nt[wv] = ntb;
+        s_nl[wv] = nlb;
+        s_rep[wv] = replayed_all;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tt = s_nt[0] + s_nt[1] + s_nt[2] + s_nt[3];
+        const uint32_t tl = s_nl[0] + s_nl[1] + s_nl[2] + s_nl[3];
+        const unsigned long long rk = s_rep[0] + s_rep[1] + s_rep[2] + s_rep[3];
+        s_base[0] = tt ? atomicAdd(&st->n_trig, tt) : 0u;
+        s_base[1] = tl ? atomicAdd(&st->n_long, tl) : 0u;
+        if (rk)
+            atomicAdd(&st->replay_keys, rk);
+    }
+    __syncthreads();
+    uint32_t bt = s_base[0], bl = s_base[1];
+    for (uint32_t w = 0; w < wv; ++w) {
+        bt += s_nt[w];
+        bl += s_nl[w];
+    }
+    write_trig(s_tr, wv, lane, ntb, bt, trig, trig_cap, st);
+    if (lane < nlb)
+        lng[bl + lane] = s_ln[wv][lane];
 }
 
 // trigger sort key: sketch << ib | stream index (fits 32 bits by construction)
