@@ -133,6 +133,19 @@ __device__ static inline void load_sid_window(const uint64_t *__restrict__ offs,
     __syncthreads();
 }
 
+// bucket and mask fields of a key's item: h % n_low and h % n_masks, or in the
+// quotient form h % n_low and h / n_low (32-bit keys: h = key * seed mod 2^32)
+template <int KB>
+__device__ static inline uint64_t bucket_mask_fields(KeyT<KB> key, const qs_sdesc &d, const qs_pack &pk)
+{
+    if (KB == 4 && pk.quot) {
+        const uint32_t h = (uint32_t)key * d.seed;
+        const uint32_t q = qs_fastquot(h, d.fd_low);
+        return (uint64_t)(h - q * d.n_low) | ((uint64_t)q << pk.mshift);
+    }
+    return (uint64_t)qs_bucket_d(key, d) | ((uint64_t)qs_maskidx_d(key, d) << pk.mshift);
+}
+
 template <int KB, bool MULTI>
 __device__ static inline uint64_t make_item(const KeyT<KB> *__restrict__ keys, uint32_t i, uint64_t base,
                                             const qs_sdesc &d0, const qs_sdesc *__restrict__ D,
@@ -141,8 +154,7 @@ __device__ static inline uint64_t make_item(const KeyT<KB> *__restrict__ keys, u
 {
     const KeyT<KB> key = keys[i];
     if (!MULTI)
-        return (uint64_t)qs_bucket_d(key, d0) | ((uint64_t)qs_maskidx_d(key, d0) << pk.mshift) |
-               ((uint64_t)i << pk.ishift);
+        return bucket_mask_fields<KB>(key, d0, pk) | ((uint64_t)i << pk.ishift);
     const uint64_t pos = base + i;
     uint32_t sid;
     uint64_t sstart;
@@ -163,8 +175,7 @@ __device__ static inline uint64_t make_item(const KeyT<KB> *__restrict__ keys, u
     }
     const qs_sdesc &d = D[sid];
     const uint64_t start = sstart > base ? sstart : base; // this sketch's part of the chunk
-    return (uint64_t)qs_bucket_d(key, d) | ((uint64_t)sid << pk.bb) |
-           ((uint64_t)qs_maskidx_d(key, d) << pk.mshift) | ((pos - start) << pk.ishift);
+    return bucket_mask_fields<KB>(key, d, pk) | ((uint64_t)sid << pk.bb) | ((pos - start) << pk.ishift);
 }
 
 template <int KB, bool FIRST, bool MULTI>
@@ -609,7 +620,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
         const uint32_t b = it_b(it, pk);
         uint32_t M = 0;
         if (valid)
-            M = mask_of<MULTI>(sid, it_mu(it, pk), D, mask_pool, s_masks, pk.unm);
+            M = mask_of<MULTI>(sid, item_mu<MULTI>(it, pk, D), D, mask_pool, s_masks, pk.unm);
         uint64_t cr = shfl64(c, lane == 63 ? 63 : (int)lane + 1);
         if (lane == 63)
             cr = (p + 1u < n) ? it_cell(IT[p + 1u], pk) : END_CELL;
@@ -659,7 +670,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg(
                 const uint64_t iq = IT[q];
                 in = it_cell(iq, pk) == c63;
                 if (in)
-                    Mq = mask_of<MULTI>(sid63, it_mu(iq, pk), D, mask_pool, s_masks, pk.unm);
+                    Mq = mask_of<MULTI>(sid63, item_mu<MULTI>(iq, pk, D), D, mask_pool, s_masks, pk.unm);
             }
             const uint64_t im = __ballot(in);
             const uint32_t head_pos = (uint32_t)w0 + s63;
@@ -768,7 +779,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             const bool v = p < n;
             const uint64_t it = v ? IT[p] : 0ull;
             c[k] = v ? it_cell(it, pk) : NO_CELL;
-            M[k] = v ? mask_of<MULTI>(it_sid(it, pk), it_mu(it, pk), D, mask_pool, s_masks, pk.unm) : 0u;
+            M[k] = v ? mask_of<MULTI>(it_sid(it, pk), item_mu<MULTI>(it, pk, D), D, mask_pool, s_masks, pk.unm) : 0u;
         }
         // cell before my first item and after my last one
         uint64_t cprev = shfl64w<W>(c[IPL - 1], (int)gl - 1);
@@ -897,7 +908,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
                 if (q < n) {
                     const uint64_t iq = IT[q];
                     if (it_cell(iq, pk) == ce) {
-                        mask_to_words(mask_of<MULTI>(sid, it_mu(iq, pk), D, mask_pool, s_masks, pk.unm), tmp);
+                        mask_to_words(mask_of<MULTI>(sid, item_mu<MULTI>(iq, pk, D), D, mask_pool, s_masks, pk.unm), tmp);
                         add_words(sum, tmp);
                         ++cnt;
                     }
@@ -1044,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
                 const uint64_t iq = IT[q];
                 valid = it_cell(iq, pk) == cell;
                 if (valid) {
-                    M = masks[it_mu(iq, pk)];
+                    M = masks[pk.quot ? qs_fastmod(it_mu(iq, pk) * d.n_low + b, d.fd_masks) : it_mu(iq, pk)];
                     idx = it_idx(iq, pk);
                 }
             }
@@ -1409,7 +1420,7 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
                     }
                     if (act) {
                         uint32_t t[7];
-                        mask_to_words(mask_of<MULTI>((uint32_t)(c >> pk.bb), it_mu(it, pk), D, mask_pool, s_masks,
+                        mask_to_words(mask_of<MULTI>((uint32_t)(c >> pk.bb), item_mu<MULTI>(it, pk, D), D, mask_pool, s_masks,
                                                      pk.unm),
                                       t);
                         add_words(acc, t);
@@ -1450,7 +1461,7 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
         if (!row)
             continue;
         const bool act = lane < len && p >= tail;
-        const uint32_t M = act ? mask_of<MULTI>((uint32_t)(ch >> pk.bb), it_mu(it, pk), D, mask_pool, s_masks, pk.unm)
+        const uint32_t M = act ? mask_of<MULTI>((uint32_t)(ch >> pk.bb), item_mu<MULTI>(it, pk, D), D, mask_pool, s_masks, pk.unm)
                                : 0u;
         uint32_t cnt = 0;
 #pragma unroll

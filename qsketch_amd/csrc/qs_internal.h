@@ -69,6 +69,15 @@ __host__ __device__ static inline uint32_t qs_fastmod(uint32_t n, const qs_fastd
     return n - q * f.d;
 }
 
+// quotient n / d of the same division
+__host__ __device__ static inline uint32_t qs_fastquot(uint32_t n, const qs_fastdiv &f)
+{
+    if (f.one)
+        return n;
+    const uint32_t t = qs_mulhi32(f.m, n);
+    return (t + ((n - t) >> 1)) >> f.sh;
+}
+
 template <int KB> struct qs_key;
 template <> struct qs_key<4> { typedef uint32_t T; };
 template <> struct qs_key<8> { typedef uint64_t T; };
@@ -113,11 +122,16 @@ __device__ static inline uint32_t qs_maskidx_d(uint64_t key, const qs_sdesc &d)
 }
 
 // One insert after hashing, in 64 bits, built once in the first radix pass:
-//   [ bucket : bb | sketch : sb | mask index : bm | stream index : ib ]
+//   [ bucket : bb | sketch : sb | mask field : bm | stream index : ib ]
 // The sort key is the low bb + sb bits (bucket within sketch), so a bucket
 // segment is a run of equal "cell" values.  The stream index is the position
 // within the sketch's part of the chunk, so allocations of one sketch can be
 // ordered by (sketch, index) in 32 bits.
+// The mask field holds the mask index h % n_masks, or (quotient form, 32-bit
+// keys only) the hash quotient q = h / n_low, from which h = q * n_low + b and
+// the mask index follow.  The quotient form is used when it is narrower: on
+// large tables (n_low = 2^27.6 needs 5 bits, 1021 masks 10), which leaves the
+// stream index room for 2^28-key chunks.
 struct qs_pack {
     uint32_t bb, sb, bm, ib;
     uint32_t mshift;   // bb + sb
@@ -125,6 +139,9 @@ struct qs_pack {
     uint32_t mmask;    // (1 << bm) - 1
     uint32_t unm;      // every sketch's mask table size when they are all equal (else 0):
                        // sketch sid's masks then start at sid * unm in the mask pool
+    uint32_t quot;     // the mask field holds the hash quotient
+    uint32_t q_low;    // a single sketch's n_low (quotient form)
+    qs_fastdiv q_fd;   // a single sketch's n_masks divisor (quotient form)
     uint64_t bmask;    // (1 << bb) - 1
     uint64_t cmask;    // (1 << (bb + sb)) - 1: the cell (sketch, bucket)
 };
@@ -134,18 +151,30 @@ static inline uint32_t qs_bits_for(uint64_t n) // bits to hold 0 .. n-1 (at leas
     return n > 1 ? 64u - (uint32_t)__builtin_clzll(n - 1) : 1u;
 }
 
-// Field widths for a set of nsk sketches whose largest table has max_low
-// buckets and max_masks masks.  The stream index gets what is left of 64 bits,
-// but at most 32 - sb bits (the allocation order key (sketch, index) is 32
-// bits) and at most 30 bits (device code shifts 32-bit values by ib; a shift
-// by 32 would wrap to a shift by 0).  Returns false when fewer than 12 bits
-// remain.
-static inline bool qs_make_pack(uint64_t max_low, uint32_t nsk, uint64_t max_masks, qs_pack *pk)
+// Field widths for a set of nsk sketches whose tables have min_low .. max_low
+// buckets and at most max_masks masks, for keys of key_bytes bytes.  The
+// stream index gets what is left of 64 bits, but at most 32 - sb bits (the
+// allocation order key (sketch, index) is 32 bits) and at most 30 bits (device
+// code shifts 32-bit values by ib; a shift by 32 would wrap to a shift by 0).
+// Returns false when fewer than 12 bits remain.  A single sketch's n_low and
+// n_masks go into the pack for the quotient form (set q_low, q_fd after).
+static inline bool qs_make_pack(uint64_t max_low, uint64_t min_low, uint32_t nsk, uint64_t max_masks,
+                                uint32_t key_bytes, qs_pack *pk)
 {
     qs_pack p;
     p.bb = qs_bits_for(max_low);
     p.sb = nsk > 1 ? qs_bits_for(nsk) : 0u;
     p.bm = qs_bits_for(max_masks);
+    p.quot = 0;
+    p.q_low = 0;
+    p.q_fd = qs_fastdiv_make(1u);
+    if (key_bytes == 4 && min_low >= 1) {
+        const uint32_t qb = qs_bits_for(0xffffffffull / min_low + 1ull); // quotients 0 .. (2^32 - 1) / min_low
+        if (qb < p.bm) {
+            p.bm = qb;
+            p.quot = 1;
+        }
+    }
     int ib = 64 - (int)(p.bb + p.sb + p.bm);
     ib = ib < 32 - (int)p.sb ? ib : 32 - (int)p.sb;
     ib = ib < 30 ? ib : 30;
@@ -181,6 +210,21 @@ __host__ __device__ static inline uint32_t it_mu(uint64_t it, const qs_pack &pk)
 __host__ __device__ static inline uint32_t it_idx(uint64_t it, const qs_pack &pk)
 {
     return (uint32_t)(it >> pk.ishift);
+}
+
+// mask index of an item: the mask field, or from the quotient form's
+// h = q * n_low + b (exact in 32 bits: it is the hash itself)
+template <bool MULTI>
+__host__ __device__ static inline uint32_t item_mu(uint64_t it, const qs_pack &pk, const qs_sdesc *D)
+{
+    const uint32_t f = it_mu(it, pk);
+    if (!pk.quot)
+        return f;
+    const uint32_t b = it_b(it, pk);
+    if (!MULTI)
+        return qs_fastmod(f * pk.q_low + b, pk.q_fd);
+    const qs_sdesc &d = D[it_sid(it, pk)];
+    return qs_fastmod(f * d.n_low + b, d.fd_masks);
 }
 
 // Device-resident control block shared by a set.

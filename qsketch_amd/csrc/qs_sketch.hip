@@ -371,16 +371,19 @@ static int create_set(const qs_params *ps, uint32_t nsk, qs_sketch **out)
         if (ps[i].key_bytes != ps[0].key_bytes)
             return QS_ERR_PARAM;
     }
-    // item packing: [bucket | sketch | mask index | stream index] in 64 bits,
+    // item packing: [bucket | sketch | mask field | stream index] in 64 bits,
     // and (sketch, stream index) in 32 bits for the allocation order
-    uint64_t max_low = 0, max_masks = 0;
+    uint64_t max_low = 0, min_low = ~0ull, max_masks = 0;
     for (uint32_t i = 0; i < nsk; ++i) {
         max_low = ps[i].n_low > max_low ? ps[i].n_low : max_low;
+        min_low = ps[i].n_low < min_low ? ps[i].n_low : min_low;
         max_masks = ps[i].n_masks > max_masks ? ps[i].n_masks : max_masks;
     }
     qs_pack pk;
-    if (!qs_make_pack(max_low, nsk, max_masks, &pk))
+    if (!qs_make_pack(max_low, min_low, nsk, max_masks, ps[0].key_bytes, &pk))
         return QS_ERR_PARAM; // too many sketches / too large tables for one set
+    pk.q_low = (uint32_t)ps[0].n_low;
+    pk.q_fd = qs_fastdiv_make(ps[0].n_masks);
     pk.unm = ps[0].n_masks;
     for (uint32_t i = 1; i < nsk; ++i)
         if (ps[i].n_masks != ps[0].n_masks)

@@ -17,12 +17,22 @@ int main()
                                   3145721, 201326557, (1ull << 29), (1ull << 31) - 1};
     std::vector<uint32_t> nsks = {1, 2, 3, 24, 64, 4096, 4097, 65536, 1u << 20};
     std::vector<uint64_t> masks = {1, 2, 61, 509, 1021, 1024, 1025, 4096};
-    long bad = 0, valid = 0, refused = 0;
+    long bad = 0, valid = 0, refused = 0, quot = 0;
+    uint64_t rng = 0x9E3779B97F4A7C15ull;
+    auto next = [&]() {
+        rng ^= rng << 13;
+        rng ^= rng >> 7;
+        rng ^= rng << 17;
+        return rng;
+    };
+    for (uint32_t kb : {4u, 8u})
     for (uint64_t L : lows)
         for (uint32_t n : nsks)
             for (uint64_t m : masks) {
                 qs_pack pk;
-                if (!qs_make_pack(L, n, m, &pk)) {
+                // a set's smallest table decides the quotient width
+                const uint64_t Lmin = n > 1 ? (L > 13 ? 13 : L) : L;
+                if (!qs_make_pack(L, Lmin, n, m, kb, &pk)) {
                     ++refused;
                     continue;
                 }
@@ -33,11 +43,46 @@ int main()
                                (unsigned long long)L, n, (unsigned long long)m, pk.bb, pk.sb, pk.bm, pk.ib);
                     ++bad;
                 };
-                if ((1ull << pk.bb) < L || (n > 1 && (1ull << pk.sb) < n) || (1ull << pk.bm) < m)
+                const uint64_t field_max = pk.quot ? 0xffffffffull / Lmin : m - 1; // largest mask-field value
+                if ((1ull << pk.bb) < L || (n > 1 && (1ull << pk.sb) < n) || (1ull << pk.bm) <= field_max)
                     fail("field too narrow");
+                if (pk.quot && (kb != 4 || (1ull << pk.bm) >= m))
+                    fail("quotient form where the mask index is as narrow");
+                if (!pk.quot && kb == 4 && (0xffffffffull / Lmin) < (1ull << (pk.bm - 1)) && pk.bm > 1)
+                    fail("mask index where the quotient is narrower");
+                if (pk.quot) {
+                    // the mask index recovered from (quotient, bucket) equals h % n_masks
+                    ++quot;
+                    pk.q_low = (uint32_t)L;
+                    pk.q_fd = qs_fastdiv_make((uint32_t)m);
+                    const qs_fastdiv fl = qs_fastdiv_make((uint32_t)L);
+                    qs_sdesc D[2] = {};
+                    D[0].n_low = (uint32_t)Lmin;
+                    D[0].fd_masks = qs_fastdiv_make((uint32_t)m);
+                    D[1].n_low = (uint32_t)L;
+                    D[1].fd_masks = qs_fastdiv_make((uint32_t)(m > 7 ? m - 6 : m));
+                    for (int r = 0; r < 200; ++r) {
+                        const uint32_t h = r < 2 ? (r ? 0xffffffffu : 0u) : (uint32_t)next();
+                        const uint32_t q = qs_fastquot(h, fl), b = h - q * (uint32_t)L;
+                        if (q != h / (uint32_t)L || b != h % (uint32_t)L)
+                            fail("quotient");
+                        const uint64_t it = b | ((uint64_t)q << pk.mshift) | (1ull << pk.ishift);
+                        if (item_mu<false>(it, pk, nullptr) != h % (uint32_t)m)
+                            fail("mask index from quotient");
+                        if (n > 1) {
+                            for (uint32_t sd = 0; sd < 2; ++sd) {
+                                const uint32_t Ls = D[sd].n_low, ms = D[sd].fd_masks.d;
+                                const uint32_t qs = h / Ls, bs = h % Ls;
+                                const uint64_t its = bs | ((uint64_t)sd << pk.bb) | ((uint64_t)qs << pk.mshift);
+                                if (item_mu<true>(its, pk, D) != h % ms)
+                                    fail("set mask index from quotient");
+                            }
+                        }
+                    }
+                }
                 if (pk.bb + pk.sb + pk.bm + pk.ib > 64 || pk.sb + pk.ib > 32 || pk.ib > 30 || pk.ib < 12)
                     fail("widths");
-                const uint64_t b = L - 1, sid = n - 1, mu = m - 1, idx = (1ull << pk.ib) - 1;
+                const uint64_t b = L - 1, sid = n - 1, mu = field_max, idx = (1ull << pk.ib) - 1;
                 const uint64_t it = b | (sid << pk.bb) | (mu << pk.mshift) | (idx << pk.ishift);
                 if (it_b(it, pk) != b || it_sid(it, pk) != sid || it_mu(it, pk) != mu || it_idx(it, pk) != idx ||
                     it_cell(it, pk) != (b | (sid << pk.bb)))
@@ -52,7 +97,7 @@ int main()
                         fail("order key monotone");
                 }
             }
-    printf("packings: %ld valid, %ld refused\n", valid, refused);
+    printf("packings: %ld valid (%ld in the quotient form), %ld refused\n", valid, quot, refused);
     printf("pack violations: %ld\n", bad);
     return bad ? 1 : 0;
 }

@@ -106,6 +106,24 @@ def test_multi_insert_matches_each_oracle(gpu, chunk_keys):
     assert st["linked_buckets"] == sum(o.next - 1025 for o in O)
 
 
+def test_multi_quotient_item_form(gpu):
+    """A set whose every table has over 2^21 buckets and 4093 masks: the items
+    carry the hash quotient and each sketch recomputes its mask index from its
+    own n_low and n_masks."""
+    _print_case(test="multi_quotient_form")
+    specs = [{"n_low": nl, "n_high": 1 << 12, "sem": i % 2, "seed": 100 + i, "alpha": a, "universe": u,
+              "n_masks": 4093}
+             for i, (nl, a, u) in enumerate([(2097169, 1.1, 1 << 12), (2200009, 0.9, 1 << 14),
+                                             (3000017, 1.2, 1 << 10)])]
+    S, O = _make(specs)
+    streams, flat, offs = _streams(specs, [200000, 150000, 250000], seed=8)
+    S.insert(to_dev(flat), offs)
+    for o, st in zip(O, streams):
+        assert o.insert(st) == 0
+    _check_set(S, O, "multi quotient form")
+    _query_set(S, O, streams)
+
+
 def test_multi_u64_keys(gpu):
     _print_case(test="multi_u64")
     specs = _specs(9, seed=8)

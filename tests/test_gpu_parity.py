@@ -67,6 +67,24 @@ def test_insert_state_bit_exact(gpu, semantics, n_low, alpha, universe, n):
     assert np.array_equal(to_host_u32(g.search(to_dev(q))), o.query(q))
 
 
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_quotient_item_form(gpu, semantics):
+    """4093 masks need a 12-bit mask field; with 2,097,169 buckets the hash
+    quotient h / n_low needs 11, so the insert items carry the quotient and the
+    mask index is recomputed from it (the form large tables use).  Several
+    inserts, chunked, with buckets linked across them."""
+    _print_case(test="quotient_form", semantics=semantics)
+    keys = zipf_keys(1.1, 1 << 16, 3 << 19, seed=21)
+    g, o = make_pair(2097169, n_high=1 << 14, n_masks=4093, semantics=semantics, chunk_keys=300000)
+    for part in np.array_split(keys, 3):
+        assert o.insert(part) == 0
+        g.insert(to_dev(part))
+    assert o.next > 1025 + 100
+    assert_same_state(g, o, "quotient form")
+    q = np.unique(keys)
+    assert np.array_equal(to_host_u32(g.search(to_dev(q))), o.query(q))
+
+
 def test_chunking_does_not_change_state(gpu):
     _print_case(test="chunking")
     keys = zipf_keys(1.2, 1 << 13, 300001, seed=5)
