@@ -178,6 +178,13 @@ def _lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -m qsketch_amd.build` "
                 "(there is no CPU fallback)")
+        try:
+            # torch ships its own HIP runtime: load it first, so the library
+            # binds to the runtime already in the process instead of loading
+            # a second one (two runtimes in one process leave torch no GPU)
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)

@@ -95,8 +95,11 @@ __device__ static inline uint32_t find_sid(const uint64_t *__restrict__ offs, ui
 // ===========================================================================
 // radix sort of the chunk by cell = (sketch, bucket)
 // ===========================================================================
-#define OS_THREADS 256
-#define OS_WAVES 4
+#ifndef QS_OS_THREADS
+#define QS_OS_THREADS 256
+#endif
+#define OS_THREADS QS_OS_THREADS
+#define OS_WAVES (OS_THREADS / 64)
 // items per lane: 8 measured best on MI355X (2^28-key Zipf insert: 16 items
 // per lane left the scatter at 4 waves/SIMD, 4 grew the scan of the tile
 // histograms more than the scatter gained)
@@ -190,7 +193,8 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
     __shared__ uint64_t s_off[MULTI ? SIDW + 1 : 1];
     __shared__ uint32_t s_w[2];
     const uint32_t t = threadIdx.x;
-    s_h[t] = 0;
+    if (t < OS_BINS)
+        s_h[t] = 0;
     const uint32_t tb = blockIdx.x * OS_TILE;
     sid_window win{s_off, 0u, 0u};
     if (FIRST && MULTI) {
@@ -318,8 +322,8 @@ __global__ __launch_bounds__(1024) void k_hist_scan_groups(uint32_t *__restrict_
         totals[d] = s_carry;
 }
 
-// exclusive block scan of one value per thread (256 threads)
-__device__ static inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_red)
+// exclusive block scan of one value per thread (OS_THREADS threads)
+__device__ static inline uint32_t block_excl_scan(uint32_t v, uint32_t *s_red)
 {
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     uint32_t x = v;
@@ -372,14 +376,15 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
         win.sid0 = s_w[0];
         win.cnt = s_w[1];
     }
+    if (t < OS_BINS)
 #pragma unroll
-    for (int w = 0; w < OS_WAVES; ++w)
-        s_wh[w][t] = 0;
+        for (int w = 0; w < OS_WAVES; ++w)
+            s_wh[w][t] = 0;
     const bool dv = t < nbins;
     const uint32_t tot = dv ? totals[t] : 0u;
     // items of digit t before this tile (tile-major counts, two-level scan)
     const uint32_t hcur = dv ? hist[(size_t)tile * nbins + t] + gsum[(size_t)(tile / SCAN_TPB) * nbins + t] : 0u;
-    const uint32_t dbase = block_excl_scan256(tot, s_red);
+    const uint32_t dbase = block_excl_scan(tot, s_red);
 
     const uint64_t lt = (1ull << lane) - 1ull;
     const int dbits = __popc(dmask);
@@ -418,16 +423,21 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     }
     __syncthreads();
     {
-        // this tile's count of digit t comes from the ranking itself
-        const uint32_t c0 = s_wh[0][t], c1 = s_wh[1][t], c2 = s_wh[2][t], c3 = s_wh[3][t];
-        const uint32_t cnt = c0 + c1 + c2 + c3;
-        s_wh[0][t] = 0;
-        s_wh[1][t] = c0;
-        s_wh[2][t] = c0 + c1;
-        s_wh[3][t] = c0 + c1 + c2;
-        const uint32_t lstart = block_excl_scan256(cnt, s_red);
-        s_ls[t] = lstart;
-        s_goff[t] = dbase + hcur - lstart;
+        // this tile's count of digit t comes from the ranking itself; the
+        // per-wave counts become per-wave starts within the digit
+        uint32_t cnt = 0;
+        if (t < OS_BINS)
+#pragma unroll
+            for (int w = 0; w < OS_WAVES; ++w) {
+                const uint32_t c = s_wh[w][t];
+                s_wh[w][t] = cnt;
+                cnt += c;
+            }
+        const uint32_t lstart = block_excl_scan(cnt, s_red);
+        if (t < OS_BINS) {
+            s_ls[t] = lstart;
+            s_goff[t] = dbase + hcur - lstart;
+        }
     }
     __syncthreads();
 #pragma unroll
