@@ -217,7 +217,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
                                                  nsk, pk, win),
                             pk);
             else
-                c = it_cell(reinterpret_cast<const uint64_t *>(in)[i], pk);
+                c = it_cell(__builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(in) + i), pk); // streamed once
             d = (uint32_t)(c >> shift) & dmask;
         }
         // one LDS add per run of equal digits on adjacent lanes (a Zipf-hot
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
                                                    nsk, pk, win)
                             : 0ull;
         else
-            item[r] = valid ? reinterpret_cast<const uint64_t *>(in)[i] : 0ull;
+            item[r] = valid ? __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(in) + i) : 0ull;
         const uint32_t d = (uint32_t)(it_cell(item[r], pk) >> shift) & dmask;
         // lanes sharing my digit: one ballot per digit bit
         uint64_t m = __ballot(valid);
