@@ -192,8 +192,9 @@ def main():
         "query_gkeys_s": round(tim["query_keys"] / (q_ms * 1e-3) / 1e9, 3) if q_ms > 0 else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": "insert path: k_bsort_up/k_sort_scan_digit/k_bsort_down per radix pass, "
-                      "k_seg, k_ordered, k_assign, k_tail",
+            "kernel": "insert path: per radix pass k_bs_up, k_hist_scan_local, k_hist_scan_groups, "
+                      "k_bs_down; then k_segw, k_ordered, k_trig_keys + pair sort, k_assign, k_tail, "
+                      "k_clear_long, k_next, k_reset",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -201,8 +202,9 @@ def main():
             "traffic": traffic,
             "traffic_source": "FETCH_SIZE/WRITE_SIZE runs of this workload calibrated by a 1 GiB copy, "
                               "profiles/insert_query_plain_zipf1.1_2e28_traffic.md" if traffic else None,
-            "launch": "the launch sequence of one insert chunk (radix passes, k_seg, k_ordered, "
-                      "k_assign, k_tail, ...), timed with HIP events around each kernel on its stream",
+            "launch": "the launch sequence of one insert chunk (radix passes, k_segw, k_ordered, "
+                      "k_assign, k_tail, ...; a 2^28-key insert is one chunk), timed with HIP events "
+                      "around each kernel on its stream",
             "bytes_per_key": INSERT_BYTES_PER_KEY,
             "keys_per_launch": round(keys_per_launch),
             "avg_launch_ms": round(ins_ms / chunks, 4),

@@ -78,9 +78,9 @@ struct qs_learned {
 // ---------------------------------------------------------------------------
 __device__ static inline uint16_t f2bf(float f)
 {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    // v_cvt_pk_bf16_f32: round to nearest even, as f2bf_host's bit arithmetic
+    // for every finite value
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 __device__ static inline float bf2f(uint16_t h)
 {
@@ -292,39 +292,62 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const KeyT<KB> *__restrict__ ke
 }
 
 // ---------------------------------------------------------------------------
-// matrix-core forward.  v_mfma_f32_32x32x8_bf16 (4 BF16 per lane):
-//   A (32 x 8):  lane l holds A[l % 32][4 (l / 32) + i],   i = 0..3
-//   B (8 x 32):  lane l holds B[4 (l / 32) + i][l % 32]
-//   D (32 x 32): lane l, register r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
-// Layer 1 computes H1^T (64 units x 32 keys) = W1 X^T as 2 row tiles x 8 K
-// steps; lane l's key is key l % 32.  The D registers of row tile t = s / 4,
-// r = 4 (s % 4) + i hold unit 8 s + 4 (l / 32) + i, which is exactly the B
-// operand of K step s of layer 2, so H1^T feeds W2 H1^T in place.
+// matrix-core forward.  v_mfma_f32_32x32x16_bf16 (8 BF16 per lane, gfx950):
+//   A (32 x 16):  lane l holds A[l % 32][8 (l / 32) + i],   i = 0..7
+//   B (16 x 32):  lane l holds B[8 (l / 32) + i][l % 32]
+//   D (32 x 32):  lane l, register r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
+// Layer 1 computes H1^T (64 units x 32 keys) = W1 X^T as 2 row tiles x 4 K
+// steps; lane l's key is key l % 32, and K step s, half h = l / 32 is key byte
+// 2 s + h.  Layer 2 sums over the units in the order the accumulators hold
+// them: K step s takes registers 8 (s % 2) .. 8 (s % 2) + 7 of row tile s / 2,
+// so slot i of half h is unit 32 (s / 2) + 8 (2 (s % 2) + i / 4) + 4 h + i % 4
+// (every unit once), and W2's columns are laid out in that order.
 // ---------------------------------------------------------------------------
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // The whole network lives in LDS: the weight operands in MFMA A-operand order
-// (layer, row tile, K step, lane -> 4 BF16), so each MFMA reads its operand
-// with one conflict-free 8-byte LDS load, plus the biases and output weights.
-// Registers then hold only accumulators and activations, for several waves
-// per SIMD.
+// (layer, row tile, K step, lane -> 8 BF16), so each MFMA reads its operand
+// with one 16-byte LDS load, plus the biases and output weights.  Registers
+// then hold only accumulators and activations, for several waves per SIMD.
 struct mlp_lds {
-    bf16x4 a[2][2][8][64];
+    bf16x8 a[2][2][4][64];
+    bf16x4 nib[16]; // layer-1 B operand of a key nibble: bit i -> 1.0 or 0.0 in BF16
     float b1[64], b2[64], w3[64], b3;
 };
 
+// max(x, 0) on the bits: negative floats are negative integers (no NaN here)
+__device__ static inline float relu_bits(float x)
+{
+    return __int_as_float(max(__float_as_int(x), 0));
+}
+
+// input column of K slot i, half h, K step s of layer 0 / 1
+__device__ static inline uint32_t kcol(uint32_t layer, uint32_t s, uint32_t h, uint32_t i)
+{
+    return layer == 0 ? 16u * s + 8u * h + i : 32u * (s / 2u) + 8u * (2u * (s % 2u) + i / 4u) + 4u * h + i % 4u;
+}
+
 __device__ static inline void load_mlp(mlp_lds &S, const uint16_t *__restrict__ pb, const float *__restrict__ p)
 {
-    for (uint32_t e = threadIdx.x; e < 2u * 2u * 8u * 64u; e += blockDim.x) {
-        const uint32_t l = e & 63u, s = (e >> 6) & 7u, t = (e >> 9) & 1u, layer = e >> 10;
+    for (uint32_t e = threadIdx.x; e < 2u * 2u * 4u * 64u; e += blockDim.x) {
+        const uint32_t l = e & 63u, s = (e >> 6) & 3u, t = (e >> 8) & 1u, layer = e >> 9;
         const uint32_t h = l >> 5, c = l & 31u;
-        const uint16_t *w = pb + (layer ? OFF_W2 : OFF_W1) + (32u * t + c) * 64u + 8u * s + 4u * h;
+        const uint16_t *w = pb + (layer ? OFF_W2 : OFF_W1) + (32u * t + c) * 64u;
+        bf16x8 v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            v[i] = (short)w[kcol(layer, s, h, (uint32_t)i)];
+        S.a[layer][t][s][l] = v;
+    }
+    for (uint32_t e = threadIdx.x; e < 16u; e += blockDim.x) {
         bf16x4 v;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            v[i] = (short)w[i];
-        S.a[layer][t][s][l] = v;
+            v[i] = ((e >> i) & 1u) ? (short)0x3f80 : (short)0;
+        S.nib[e] = v;
     }
     for (uint32_t e = threadIdx.x; e < 64u; e += blockDim.x) {
         S.b1[e] = p[OFF_B1 + e];
@@ -342,6 +365,12 @@ __device__ static inline uint32_t drow(int t, int r, uint32_t h)
     return 32u * t + 8u * (r / 4) + 4u * h + (r % 4);
 }
 
+__device__ static inline f32x16 mfma16(const bf16x8 &a, const bf16x8 &b, const f32x16 &c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, b), c,
+                                                   0, 0, 0);
+}
+
 // score of key x (lane l scores key l % 32; both half-waves return it)
 __device__ static inline float mlp_mfma(const mlp_lds &S, uint64_t x)
 {
@@ -355,22 +384,22 @@ __device__ static inline float mlp_mfma(const mlp_lds &S, uint64_t x)
         acc0[r] = S.b1[drow(0, r, h)];
         acc1[r] = S.b1[drow(1, r, h)];
     }
+    const uint64_t xs = x >> (8u * h); // this half-wave's bytes: byte 2 s + h at bits 16 s
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        bf16x4 b;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            b[i] = ((x >> (8 * s + 4 * h + i)) & 1ull) ? (short)0x3f80 : (short)0;
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(S.a[0][0][s][lane], b, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(S.a[0][1][s][lane], b, acc1, 0, 0, 0);
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t byte = (uint32_t)(xs >> (16 * s)) & 0xffu;
+        const bf16x4 lo = S.nib[byte & 0xfu], hi = S.nib[byte >> 4];
+        const bf16x8 b = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        acc0 = mfma16(S.a[0][0][s][lane], b, acc0);
+        acc1 = mfma16(S.a[0][1][s][lane], b, acc1);
     }
-    bf16x4 hb[8];
+    bf16x8 hb[4];
 #pragma unroll
-    for (int s = 0; s < 8; ++s)
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = 4 * (s % 4) + i;
-            hb[s][i] = (short)f2bf(fmaxf(s < 4 ? acc0[r] : acc1[r], 0.f));
+        for (int i = 0; i < 8; ++i) {
+            const int r = 8 * (s % 2) + i;
+            hb[s][i] = (short)f2bf(relu_bits(s < 2 ? acc0[r] : acc1[r]));
         }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -378,15 +407,15 @@ __device__ static inline float mlp_mfma(const mlp_lds &S, uint64_t x)
         acc1[r] = S.b2[drow(1, r, h)];
     }
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(S.a[1][0][s][lane], hb[s], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(S.a[1][1][s][lane], hb[s], acc1, 0, 0, 0);
+    for (int s = 0; s < 4; ++s) {
+        acc0 = mfma16(S.a[1][0][s][lane], hb[s], acc0);
+        acc1 = mfma16(S.a[1][1][s][lane], hb[s], acc1);
     }
     float y = 0.f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        y = fmaf(S.w3[drow(0, r, h)], bf2f(f2bf(fmaxf(acc0[r], 0.f))), y);
-        y = fmaf(S.w3[drow(1, r, h)], bf2f(f2bf(fmaxf(acc1[r], 0.f))), y);
+        y = fmaf(S.w3[drow(0, r, h)], bf2f(f2bf(relu_bits(acc0[r]))), y);
+        y = fmaf(S.w3[drow(1, r, h)], bf2f(f2bf(relu_bits(acc1[r]))), y);
     }
     y += __shfl_xor(y, 32, 64);
     return y + S.b3;
