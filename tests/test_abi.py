@@ -56,8 +56,12 @@ def test_primes_match_oracle():
 def test_error_strings_and_bad_params():
     import qsketch_amd as q
     L = q.lib()
-    for code in (0, -1, -2, -3, -4, -5, -6, -99):
+    for code in (0, -1, -2, -3, -4, -5, -6, -7, -8, -99):
         assert L.qs_strerror(code)
+    # every code has its own message; unknown codes share one
+    msgs = [L.qs_strerror(c) for c in (0, -1, -2, -3, -4, -5, -6, -7, -8)]
+    assert len(set(msgs)) == len(msgs) and L.qs_strerror(-99) not in msgs
+    assert q.QS_ERR_INTERNAL == -8 and b"consistency" in L.qs_strerror(q.QS_ERR_INTERNAL)
     p = q.default_params(1 << 20)
     h = C.c_void_p()
     for field, bad in (("m_low", 4), ("m_high", 16), ("mask_ones", 2), ("link_start", 7),
