@@ -187,7 +187,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
                                                       const uint64_t *__restrict__ offs, uint32_t nsk,
                                                       qs_pack pk, int shift, uint32_t dmask,
                                                       uint32_t nbins, uint32_t ntiles,
-                                                      uint32_t *__restrict__ hist)
+                                                      uint16_t *__restrict__ hist)
 {
     __shared__ uint32_t s_h[OS_BINS];
     __shared__ uint64_t s_off[MULTI ? SIDW + 1 : 1];
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
     }
     __syncthreads();
     if (t < nbins)
-        hist[(size_t)blockIdx.x * nbins + t] = s_h[t]; // tile-major: one contiguous row per tile
+        hist[(size_t)blockIdx.x * nbins + t] = (uint16_t)s_h[t]; // tile-major: one contiguous row per tile
 }
 
 // Two-level exclusive scan over tiles of the tile-major digit counts.
@@ -243,8 +243,11 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
 // the group's sums.  Level 2: one block per digit scans that digit's group
 // sums and writes its total.  The scatter adds the two levels.
 #define SCAN_TPB 16u
+// The tile counts and their group-local prefixes are 16-bit: a count is at
+// most OS_TILE and a prefix at most (SCAN_TPB - 1) * OS_TILE.
+static_assert(SCAN_TPB * OS_TILE <= 65536u, "16-bit tile counts");
 
-__global__ __launch_bounds__(256) void k_hist_scan_local(uint32_t *__restrict__ hist, uint32_t ntiles,
+__global__ __launch_bounds__(256) void k_hist_scan_local(uint16_t *__restrict__ hist, uint32_t ntiles,
                                                          uint32_t nbins, uint32_t *__restrict__ gsum)
 {
     const uint32_t d = threadIdx.x;
@@ -255,9 +258,9 @@ __global__ __launch_bounds__(256) void k_hist_scan_local(uint32_t *__restrict__ 
     uint32_t run = 0;
 #pragma unroll 8
     for (uint32_t t = t0; t < t1; ++t) {
-        uint32_t *h = hist + (size_t)t * nbins + d;
+        uint16_t *h = hist + (size_t)t * nbins + d;
         const uint32_t v = *h;
-        *h = run;
+        *h = (uint16_t)run;
         run += v;
     }
     gsum[(size_t)blockIdx.x * nbins + d] = run;
@@ -352,7 +355,7 @@ template <int KB, bool FIRST, bool MULTI>
 __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const void *__restrict__ in, uint32_t n, uint64_t base, qs_sdesc d0, const qs_sdesc *__restrict__ D,
     const uint64_t *__restrict__ offs, uint32_t nsk, qs_pack pk, int shift, uint32_t dmask,
-    uint32_t nbins, uint32_t ntiles, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ gsum,
+    uint32_t nbins, uint32_t ntiles, const uint16_t *__restrict__ hist, const uint32_t *__restrict__ gsum,
     const uint32_t *__restrict__ totals, uint64_t *__restrict__ out, qs_dev_state *__restrict__ st)
 {
     __shared__ uint64_t s_item[OS_TILE];
@@ -1600,6 +1603,7 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
     const int dbits = (bits + passes - 1) / passes; // <= 8
     const uint32_t nbins = 1u << dbits, dmask = nbins - 1u;
     const uint32_t ngrp = (ntiles + SCAN_TPB - 1) / SCAN_TPB;
+    uint16_t *tcount = reinterpret_cast<uint16_t *>(s->d_hist); // 16-bit tile counts (half the region)
     uint32_t *gsum = s->d_hist + (size_t)OS_BINS * ntiles;
     uint32_t *totals = gsum + (size_t)OS_BINS * ngrp;
     const qs_sdesc d0 = s->h_desc[0];
@@ -1613,16 +1617,16 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
         if (ps == 0)
             hipLaunchKernelGGL((k_bs_up<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               s->d_hist);
+                               tcount);
         else
             hipLaunchKernelGGL((k_bs_up<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               s->d_hist);
+                               tcount);
         if (bc.timed) {
             qs_tim_end(s, e0, stream);
             qs_tim_begin(s, e1, QS_TK_SORT_SCAN, cn, stream);
         }
-        hipLaunchKernelGGL(k_hist_scan_local, dim3(ngrp), dim3(256), 0, stream, s->d_hist, ntiles, nbins, gsum);
+        hipLaunchKernelGGL(k_hist_scan_local, dim3(ngrp), dim3(256), 0, stream, tcount, ntiles, nbins, gsum);
         hipLaunchKernelGGL(k_hist_scan_groups, dim3(nbins), dim3(1024), 0, stream, gsum, ngrp, nbins, totals);
         if (bc.timed) {
             qs_tim_end(s, e1, stream);
@@ -1631,11 +1635,11 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
         if (ps == 0)
             hipLaunchKernelGGL((k_bs_down<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               s->d_hist, gsum, totals, s->d_it[out], s->d_st);
+                               tcount, gsum, totals, s->d_it[out], s->d_st);
         else
             hipLaunchKernelGGL((k_bs_down<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               s->d_hist, gsum, totals, s->d_it[out], s->d_st);
+                               tcount, gsum, totals, s->d_it[out], s->d_st);
         if (bc.timed)
             qs_tim_end(s, e2, stream);
         in = s->d_it[out];
