@@ -782,25 +782,54 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
     const uint64_t gmask = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (lane - gl));
     const uint64_t gid = ((uint64_t)blockIdx.x * SEG_THREADS + threadIdx.x) / W;
     const uint64_t ngroups = (uint64_t)gridDim.x * SEG_THREADS / W;
+    // the next window's items (and the items either side of it) are loaded
+    // while this window is processed
+    uint64_t nit[IPL], nedge = 0;
+    {
+        const uint64_t w0 = gid * T, p0 = w0 + (uint64_t)gl * IPL;
+#pragma unroll
+        for (int k = 0; k < IPL; ++k)
+            nit[k] = p0 + k < n ? IT[p0 + k] : 0ull;
+        if (gl == 0 && w0 > 0 && w0 < n)
+            nedge = IT[w0 - 1];
+        if (gl == (uint32_t)(W - 1) && w0 + T < n)
+            nedge = IT[w0 + T];
+    }
     for (uint64_t w0 = gid * T; w0 < n; w0 += ngroups * T) {
         const uint64_t p0 = w0 + (uint64_t)gl * IPL;
         uint64_t c[IPL];
         uint32_t M[IPL];
+        uint64_t itk[IPL];
+        const uint64_t edge = nedge;
+#pragma unroll
+        for (int k = 0; k < IPL; ++k)
+            itk[k] = nit[k];
+        {
+            const uint64_t w1 = w0 + ngroups * T, p1 = w1 + (uint64_t)gl * IPL;
+#pragma unroll
+            for (int k = 0; k < IPL; ++k)
+                nit[k] = p1 + k < n ? IT[p1 + k] : 0ull;
+            nedge = 0;
+            if (gl == 0 && w1 < n)
+                nedge = IT[w1 - 1];
+            if (gl == (uint32_t)(W - 1) && w1 + T < n)
+                nedge = IT[w1 + T];
+        }
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
             const uint64_t p = p0 + k;
             const bool v = p < n;
-            const uint64_t it = v ? IT[p] : 0ull;
+            const uint64_t it = itk[k];
             c[k] = v ? it_cell(it, pk) : NO_CELL;
             M[k] = v ? mask_of<MULTI>(it_sid(it, pk), item_mu<MULTI>(it, pk, D), D, mask_pool, s_masks, pk.unm) : 0u;
         }
         // cell before my first item and after my last one
         uint64_t cprev = shfl64w<W>(c[IPL - 1], (int)gl - 1);
         if (gl == 0)
-            cprev = (w0 > 0) ? it_cell(IT[w0 - 1], pk) : NO_CELL;
+            cprev = (w0 > 0) ? it_cell(edge, pk) : NO_CELL;
         uint64_t cnext = shfl64w<W>(c[0], (int)gl + 1);
         if (gl == (uint32_t)(W - 1))
-            cnext = (w0 + T < n) ? it_cell(IT[w0 + T], pk) : END_CELL;
+            cnext = (w0 + T < n) ? it_cell(edge, pk) : END_CELL;
         const bool head0 = c[0] != NO_CELL && c[0] != cprev;
         bool anyhead = head0;
 #pragma unroll
