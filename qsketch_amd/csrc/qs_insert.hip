@@ -47,6 +47,7 @@
 #include "qs_internal.h"
 
 #include <cstring>
+#include <type_traits>
 
 #define CK(x)                                                                                    \
     do {                                                                                         \
@@ -150,12 +151,11 @@ __device__ static inline uint64_t bucket_mask_fields(KeyT<KB> key, const qs_sdes
 }
 
 template <int KB, bool MULTI>
-__device__ static inline uint64_t make_item(const KeyT<KB> *__restrict__ keys, uint32_t i, uint64_t base,
+__device__ static inline uint64_t make_item(const KeyT<KB> key, uint32_t i, uint64_t base,
                                             const qs_sdesc &d0, const qs_sdesc *__restrict__ D,
                                             const uint64_t *__restrict__ offs, uint32_t nsk,
                                             const qs_pack &pk, const sid_window &win)
 {
-    const KeyT<KB> key = keys[i];
     if (!MULTI)
         return bucket_mask_fields<KB>(key, d0, pk) | ((uint64_t)i << pk.ishift);
     const uint64_t pos = base + i;
@@ -179,6 +179,17 @@ __device__ static inline uint64_t make_item(const KeyT<KB> *__restrict__ keys, u
     const qs_sdesc &d = D[sid];
     const uint64_t start = sstart > base ? sstart : base; // this sketch's part of the chunk
     return bucket_mask_fields<KB>(key, d, pk) | ((uint64_t)sid << pk.bb) | ((pos - start) << pk.ishift);
+}
+
+// what a radix pass reads per position: the key (first pass) or the item,
+// streamed once (non-temporal); positions past n read the last one
+template <int KB, bool FIRST> using RawT = typename std::conditional<FIRST, KeyT<KB>, uint64_t>::type;
+
+template <int KB, bool FIRST>
+__device__ static inline RawT<KB, FIRST> load_raw(const void *in, uint32_t i, uint32_t n)
+{
+    const uint32_t ic = i < n ? i : n - 1u;
+    return __builtin_nontemporal_load(reinterpret_cast<const RawT<KB, FIRST> *>(in) + ic);
 }
 
 template <int KB, bool FIRST, bool MULTI>
@@ -205,7 +216,13 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
     }
     __syncthreads();
     const uint32_t lane = t & 63u;
-#pragma unroll 4
+    // every load first, from clamped positions: a load inside a per-item branch
+    // is issued and waited for one at a time
+    RawT<KB, FIRST> raw[OS_IPL];
+#pragma unroll
+    for (int r = 0; r < OS_IPL; ++r)
+        raw[r] = load_raw<KB, FIRST>(in, tb + r * OS_THREADS + t, n);
+#pragma unroll
     for (int r = 0; r < OS_IPL; ++r) {
         const uint32_t i = tb + r * OS_THREADS + t; // a wavefront's lanes hold consecutive items
         const bool valid = i < n;
@@ -213,11 +230,9 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
         if (valid) {
             uint64_t c;
             if (FIRST)
-                c = it_cell(make_item<KB, MULTI>(reinterpret_cast<const KeyT<KB> *>(in), i, base, d0, D, offs,
-                                                 nsk, pk, win),
-                            pk);
+                c = it_cell(make_item<KB, MULTI>((KeyT<KB>)raw[r], i, base, d0, D, offs, nsk, pk, win), pk);
             else
-                c = it_cell(__builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(in) + i), pk); // streamed once
+                c = it_cell((uint64_t)raw[r], pk);
             d = (uint32_t)(c >> shift) & dmask;
         }
         // one LDS add per run of equal digits on adjacent lanes (a Zipf-hot
@@ -390,29 +405,36 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const uint32_t dbase = block_excl_scan(tot, s_red);
 
     const uint64_t lt = (1ull << lane) - 1ull;
-    const int dbits = __popc(dmask);
     uint64_t item[OS_IPL];
     uint32_t dr[OS_IPL]; // digit | wave-local position << 8 | valid << 31
+    {
+        // every load first (see k_bs_up)
+        RawT<KB, FIRST> raw[OS_IPL];
+#pragma unroll
+        for (int r = 0; r < OS_IPL; ++r)
+            raw[r] = load_raw<KB, FIRST>(in, tb + wv * OS_WAVE_ITEMS + r * 64u + lane, n);
+#pragma unroll
+        for (int r = 0; r < OS_IPL; ++r) {
+            const uint32_t i = tb + wv * OS_WAVE_ITEMS + r * 64u + lane;
+            if (FIRST)
+                item[r] = i < n ? make_item<KB, MULTI>((KeyT<KB>)raw[r], i, base, d0, D, offs, nsk, pk, win) : 0ull;
+            else
+                item[r] = i < n ? (uint64_t)raw[r] : 0ull;
+        }
+    }
 #pragma unroll
     for (int r = 0; r < OS_IPL; ++r) {
         const uint32_t i = tb + wv * OS_WAVE_ITEMS + r * 64u + lane;
         const bool valid = i < n;
-        if (FIRST)
-            item[r] = valid ? make_item<KB, MULTI>(reinterpret_cast<const KeyT<KB> *>(in), i, base, d0, D, offs,
-                                                   nsk, pk, win)
-                            : 0ull;
-        else
-            item[r] = valid ? __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(in) + i) : 0ull;
         const uint32_t d = (uint32_t)(it_cell(item[r], pk) >> shift) & dmask;
-        // lanes sharing my digit: one ballot per digit bit
+        // lanes sharing my digit: one ballot per digit bit (the bits above the
+        // digit are 0 in every lane and leave m as it is)
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; ++bit) {
-            if (bit < dbits) {
-                const bool on = (d >> bit) & 1u;
-                const uint64_t bb = __ballot(on);
-                m &= on ? bb : ~bb;
-            }
+            const bool on = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(on);
+            m &= on ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t)__popcll(m & lt);
         const int leader = valid ? (int)__builtin_ctzll(m) : 0;
