@@ -804,18 +804,22 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
     const uint64_t gmask = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (lane - gl));
     const uint64_t gid = ((uint64_t)blockIdx.x * SEG_THREADS + threadIdx.x) / W;
     const uint64_t ngroups = (uint64_t)gridDim.x * SEG_THREADS / W;
-    // the next window's items (and the items either side of it) are loaded
-    // while this window is processed
-    uint64_t nit[IPL], nedge = 0;
+    // The next window's items (and, in the first and last lane of the group,
+    // the items either side of it) are loaded while this window is processed.
+    // The loads are unconditional, from clamped positions, so that the wait
+    // for this window's items does not also wait for them.
+    uint64_t nit[IPL], nedge;
+    const uint64_t last = (uint64_t)n - 1u;
+    auto edge_pos = [&](uint64_t w) -> uint64_t {
+        const uint64_t e = gl == 0 ? (w > 0 ? w - 1u : 0u) : w + T; // only lanes 0 and W - 1 use it
+        return e < last ? e : last;
+    };
     {
         const uint64_t w0 = gid * T, p0 = w0 + (uint64_t)gl * IPL;
 #pragma unroll
         for (int k = 0; k < IPL; ++k)
-            nit[k] = p0 + k < n ? IT[p0 + k] : 0ull;
-        if (gl == 0 && w0 > 0 && w0 < n)
-            nedge = IT[w0 - 1];
-        if (gl == (uint32_t)(W - 1) && w0 + T < n)
-            nedge = IT[w0 + T];
+            nit[k] = IT[p0 + k < last ? p0 + k : last];
+        nedge = IT[edge_pos(w0)];
     }
     for (uint64_t w0 = gid * T; w0 < n; w0 += ngroups * T) {
         const uint64_t p0 = w0 + (uint64_t)gl * IPL;
@@ -830,12 +834,8 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             const uint64_t w1 = w0 + ngroups * T, p1 = w1 + (uint64_t)gl * IPL;
 #pragma unroll
             for (int k = 0; k < IPL; ++k)
-                nit[k] = p1 + k < n ? IT[p1 + k] : 0ull;
-            nedge = 0;
-            if (gl == 0 && w1 < n)
-                nedge = IT[w1 - 1];
-            if (gl == (uint32_t)(W - 1) && w1 + T < n)
-                nedge = IT[w1 + T];
+                nit[k] = IT[p1 + k < last ? p1 + k : last];
+            nedge = IT[edge_pos(w1)];
         }
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
@@ -1395,11 +1395,14 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
         uint32_t c_tail = 0;
         // cell of the item before the step
         uint64_t c_prev = (w0 > 0) ? it_cell(IT[w0 - 1u], pk) : NO_CELL;
+        // unconditional loads from clamped positions (validity is applied at
+        // use), so that waiting for one group does not wait for the next
+        const uint64_t last = (uint64_t)n - 1u;
         uint64_t buf[TAIL_PF];
 #pragma unroll
         for (int k = 0; k < TAIL_PF; ++k) {
             const uint64_t q = w0 + 64u * k + lane;
-            buf[k] = q < wend ? IT[q] : 0ull;
+            buf[k] = IT[q < last ? q : last];
         }
         for (uint64_t g = w0; g < wend; g += 64u * TAIL_PF) {
             // the next TAIL_PF steps' items are loaded while these are processed
@@ -1407,7 +1410,7 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
 #pragma unroll
             for (int k = 0; k < TAIL_PF; ++k) {
                 const uint64_t q = g + 64u * (TAIL_PF + k) + lane;
-                nb[k] = q < wend ? IT[q] : 0ull;
+                nb[k] = IT[q < last ? q : last];
             }
 #pragma unroll
             for (int k = 0; k < TAIL_PF; ++k) {
