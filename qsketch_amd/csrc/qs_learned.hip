@@ -94,6 +94,16 @@ static inline uint16_t f2bf_host(float f)
     return (uint16_t)(u >> 16);
 }
 
+// the key at i, or at n - 1 past the end (n > 0): loads that prefetch the next
+// tile's keys stay unconditional, so the wait for this tile's keys does not also
+// wait for them
+template <int KB> __device__ static inline uint64_t key64(const KeyT<KB> *keys, uint64_t i);
+template <int KB>
+__device__ static inline uint64_t key_at(const KeyT<KB> *keys, uint64_t i, uint64_t n)
+{
+    return key64<KB>(keys, i < n ? i : n - 1u);
+}
+
 template <int KB>
 __device__ static inline uint64_t key64(const KeyT<KB> *keys, uint64_t i)
 {
@@ -431,12 +441,14 @@ __global__ __launch_bounds__(256) void k_mlp_mfma(const KeyT<KB> *__restrict__ k
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t gw = ((uint64_t)blockIdx.x * 256u + threadIdx.x) >> 6;
     const uint64_t nw = ((uint64_t)gridDim.x * 256u) >> 6;
+    if (n == 0)
+        return;
     uint64_t i = gw * 32u + (lane & 31u);
-    uint64_t x = i < n ? key64<KB>(keys, i) : 0ull;
+    uint64_t x = key_at<KB>(keys, i, n);
     for (uint64_t base = gw * 32u; base < n; base += nw * 32u) {
         // the next key is loaded while this one is scored
         const uint64_t i2 = i + nw * 32u;
-        const uint64_t x2 = i2 < n ? key64<KB>(keys, i2) : 0ull;
+        const uint64_t x2 = key_at<KB>(keys, i2, n);
         const float y = mlp_mfma(S, x);
         if (i < n && lane < 32u)
             out[i] = y;
@@ -523,7 +535,7 @@ __global__ __launch_bounds__(ROUTE_THREADS) void k_route(const KeyT<KB> *__restr
     uint64_t xn = 0;
     {
         const uint64_t i0 = tb + (uint64_t)wv * 32u + (lane & 31u);
-        xn = i0 < n ? key64<KB>(keys, i0) : 0ull;
+        xn = key_at<KB>(keys, i0, n);
     }
     for (uint32_t st = 0; st < ROUTE_TILE / 128u; ++st) {
         const uint64_t i = tb + (uint64_t)(st * 4u + wv) * 32u + (lane & 31u);
@@ -531,7 +543,7 @@ __global__ __launch_bounds__(ROUTE_THREADS) void k_route(const KeyT<KB> *__restr
         const uint64_t x = xn;
         // the next step's key is loaded while this one is scored
         const uint64_t in = i + 128u;
-        xn = (st + 1 < ROUTE_TILE / 128u && in < n) ? key64<KB>(keys, in) : 0ull;
+        xn = key_at<KB>(keys, in, n);
         const float y = mlp_mfma(S, x);
         if (lane >= 32u || !valid)
             continue;
@@ -609,10 +621,14 @@ __global__ __launch_bounds__(256) void k_route_query(const KeyT<KB> *__restrict_
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t gw = ((uint64_t)blockIdx.x * 256u + threadIdx.x) >> 6;
     const uint64_t nw = ((uint64_t)gridDim.x * 256u) >> 6;
+    if (n == 0)
+        return;
+    uint64_t xn = key_at<KB>(keys, gw * 32u + (lane & 31u), n);
     for (uint64_t base = gw * 32u; base < n; base += nw * 32u) {
         const uint64_t i = base + (lane & 31u);
         const bool valid = i < n;
-        const uint64_t x = valid ? key64<KB>(keys, i) : 0ull;
+        const uint64_t x = xn;
+        xn = key_at<KB>(keys, i + nw * 32u, n); // the next tile's key, loaded while this one is scored
         const float y = mlp_mfma(S, x);
         if (lane < 32u && valid && y >= thr && x != T_EMPTY) {
             const uint32_t s = t_find(tkeys, tlog2, x);
