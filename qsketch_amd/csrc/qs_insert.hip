@@ -1349,7 +1349,20 @@ __device__ static inline uint32_t *tail_row(uint32_t sid, uint32_t b, const qs_s
     return high_pool + high_off + (size_t)(w - QS_LINK_START) * 32u;
 }
 
-template <bool MULTI>
+// a cell shuffle in the cell's own width
+template <typename CT> __device__ static inline CT shfl_cell(CT v, int src);
+template <> __device__ inline uint32_t shfl_cell<uint32_t>(uint32_t v, int src)
+{
+    return (uint32_t)__shfl((int)v, src, 64);
+}
+template <> __device__ inline uint64_t shfl_cell<uint64_t>(uint64_t v, int src)
+{
+    return shfl64(v, src);
+}
+
+// CT: the cell type, uint32_t when bucket and sketch bits fit in 31 (then the
+// sentinels NO_CELL, END_CELL truncated to 32 bits are above every cell)
+template <bool MULTI, typename CT>
 __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, uint32_t n,
                                               const qs_sdesc *__restrict__ D, uint32_t n_masks0,
                                               const uint32_t *__restrict__ low_pool,
@@ -1380,11 +1393,11 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
         const uint64_t wend = (w0 + TAIL_WINDOW < n) ? w0 + TAIL_WINDOW : n;
         uint32_t *cur = nullptr;
         // lookup cache: the last cell whose row was fetched
-        uint64_t c_c = NO_CELL;
+        CT c_c = (CT)NO_CELL;
         uint32_t *c_row = nullptr;
         uint32_t c_tail = 0;
         // cell of the item before the step
-        uint64_t c_prev = (w0 > 0) ? it_cell(IT[w0 - 1u], pk) : NO_CELL;
+        CT c_prev = (w0 > 0) ? (CT)it_cell(IT[w0 - 1u], pk) : (CT)NO_CELL;
         // unconditional loads from clamped positions (validity is applied at
         // use), so that waiting for one group does not wait for the next
         const uint64_t last = (uint64_t)n - 1u;
@@ -1411,14 +1424,15 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
                 const bool valid = p < wend;
                 const uint64_t it = buf[k];
                 const uint64_t nx = (k + 1 < TAIL_PF) ? buf[k + 1] : nb[0]; // the next step's items
-                const uint64_t c = valid ? it_cell(it, pk) : NO_CELL;
+                const CT c = valid ? (CT)it_cell(it, pk) : (CT)NO_CELL;
                 const uint32_t last_lane = (wend - s0 < 64u) ? (uint32_t)(wend - s0 - 1u) : 63u;
-                const uint64_t c_first = shfl64(c, 0);
-                const uint64_t c_last = shfl64(c, (int)last_lane);
-                const uint64_t nx0 = shfl64(nx, 0); // taken by the whole wavefront
-                const uint64_t c_after = (s0 + 64u < wend) ? it_cell(nx0, pk)
-                                         : (wend < n)      ? it_cell(IT[wend], pk)
-                                                           : END_CELL;
+                const CT c_first = shfl_cell<CT>(c, 0);
+                const CT c_last = shfl_cell<CT>(c, (int)last_lane);
+                const CT nxc = (CT)it_cell(nx, pk);
+                const CT nx0 = shfl_cell<CT>(nxc, 0); // taken by the whole wavefront
+                const CT c_after = (s0 + 64u < wend) ? nx0
+                                   : (wend < n)      ? (CT)it_cell(IT[wend], pk)
+                                                     : (CT)END_CELL;
                 // This pass takes the runs that cross a step boundary (hold two
                 // positions 64k - 1 and 64k); in every step such a run touches it
                 // holds the first or the last item.  Runs inside one step are
@@ -1447,7 +1461,7 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
                         bool need = true;
                         if (!short_runs && s0 + 64u < wend) {
                             const bool nv = s0 + 64u + lane < wend;
-                            const uint64_t nm = __ballot(nv && it_cell(nx, pk) == c_last);
+                            const uint64_t nm = __ballot(nv && nxc == c_last);
                             const uint32_t b = (~nm) ? (uint32_t)__builtin_ctzll(~nm) : 64u;
                             const uint32_t a = (uint32_t)__popcll(__ballot(valid && c == c_last));
                             need = a + b > 64u;
@@ -1785,10 +1799,15 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
     }
     // k_segw windows under 64 items mark runs long that fit inside one step
     const bool short_runs = s->seg_w != 0 && s->seg_w * s->seg_ipl < 64u;
-    hipLaunchKernelGGL(k_tail<MULTI>,
-                       dim3(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8)), dim3(256), 0,
-                       stream, IT, cn, s->d_desc, nm0, s->d_low, s->d_high, s->d_masks, s->pk, s->d_longbm,
-                       s->d_tail_of, s->d_next, s->d_long, short_runs, s->d_st);
+    const dim3 tgrid(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8));
+    if (s->pk.bb + s->pk.sb <= 31u)
+        hipLaunchKernelGGL((k_tail<MULTI, uint32_t>), tgrid, dim3(256), 0, stream, IT, cn, s->d_desc, nm0, s->d_low,
+                           s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, s->d_long, short_runs,
+                           s->d_st);
+    else
+        hipLaunchKernelGGL((k_tail<MULTI, uint64_t>), tgrid, dim3(256), 0, stream, IT, cn, s->d_desc, nm0, s->d_low,
+                           s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, s->d_long, short_runs,
+                           s->d_st);
     hipLaunchKernelGGL(k_clear_long, dim3(grid_cap(((uint64_t)cn / 64 + 255) / 256, 4)), dim3(256), 0, stream,
                        s->d_long, IT, s->d_desc, s->pk, s->d_longbm, s->d_st);
     hipLaunchKernelGGL(k_next, dim3(grid_cap(((uint64_t)s->trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
