@@ -1087,19 +1087,29 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
                                                  uint32_t *__restrict__ longbm,
                                                  uint32_t *__restrict__ lng,
                                                  qs_trig *__restrict__ trig, uint32_t trig_cap,
-                                                 qs_dev_state *__restrict__ st)
+                                                 uint32_t one_masks, qs_dev_state *__restrict__ st)
 {
     __shared__ qs_trig s_tr[4][TBUF];
     __shared__ uint32_t s_ln[4][TBUF];
+    __shared__ uint32_t s_masks[QS_MAX_MASKS]; // a single sketch's masks (one_masks of them)
+    for (uint32_t i = threadIdx.x; i < one_masks; i += blockDim.x)
+        s_masks[i] = mask_pool[i];
+    __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t gw = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * (blockDim.x / 64u);
     const uint32_t n_ord = st->n_ord;
     uint32_t ntb = 0, nlb = 0;
     unsigned long long replayed_all = 0;
+    // The next segment's head and the next batch's items are loaded ahead,
+    // unconditionally from clamped positions, so that waits do not cover them.
+    const uint32_t last = n - 1u, olast = n_ord ? n_ord - 1u : 0u;
+    uint32_t p0n = ord[gw < olast ? gw : olast];
     for (uint32_t h = gw; h < n_ord; h += nw) {
-        const uint32_t p0 = ord[h];
-        const uint64_t cell = it_cell(IT[p0], pk);
+        const uint32_t p0 = p0n;
+        p0n = ord[h + nw < olast ? h + nw : olast];
+        uint64_t iq = IT[p0 + lane < last ? p0 + lane : last]; // the first batch; lane 0 holds the head
+        const uint64_t cell = it_cell(shfl64(iq, 0), pk);
         const uint32_t b = (uint32_t)(cell & pk.bmask);
         const uint32_t sid = (uint32_t)(cell >> pk.bb);
         const qs_sdesc &d = D[sid];
@@ -1112,16 +1122,15 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
         bool fired = false, more = false;
         uint32_t t_idx = 0, tail_start = 0, replayed = 0;
         for (uint32_t p = p0;; p += 64u) {
+            const uint64_t iqc = iq; // this batch's item
+            iq = IT[p + 64u + lane < last ? p + 64u + lane : last];
             const uint32_t q = p + lane;
-            bool valid = false;
+            const bool valid = q < n && it_cell(iqc, pk) == cell;
             uint32_t M = 0, idx = 0;
-            if (q < n) {
-                const uint64_t iq = IT[q];
-                valid = it_cell(iq, pk) == cell;
-                if (valid) {
-                    M = masks[pk.quot ? qs_fastmod(it_mu(iq, pk) * d.n_low + b, d.fd_masks) : it_mu(iq, pk)];
-                    idx = it_idx(iq, pk);
-                }
+            if (valid) {
+                const uint32_t mu = pk.quot ? qs_fastmod(it_mu(iqc, pk) * d.n_low + b, d.fd_masks) : it_mu(iqc, pk);
+                M = one_masks ? s_masks[mu] : masks[mu];
+                idx = it_idx(iqc, pk);
             }
             const uint64_t vm = __ballot(valid);
             if (vm == 0)
@@ -1778,7 +1787,7 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
     }
     hipLaunchKernelGGL(k_ordered, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(256), 0, stream, IT, cn,
                        s->d_ord, s->d_desc, s->d_low, s->d_masks, s->pk, s->d_longbm, s->d_long, s->d_trig,
-                       s->trig_cap, s->d_st);
+                       s->trig_cap, s->nsk == 1 ? s->h_desc[0].n_masks : 0u, s->d_st);
     if (bc.timed) {
         qs_tim_end(s, eo, stream);
         qs_tim_begin(s, ea, QS_TK_ASSIGN, cn, stream);
