@@ -27,8 +27,11 @@
 //
 // Per chunk (a fixed launch sequence with no host synchronisation, so it can
 // be captured in a hipGraph; all counts stay on the device):
-//   sort      per pass: k_bs_up (tile digit counts), a scan over tiles,
-//             k_bs_down (rank, stage digit-sorted in LDS, write digit runs)
+//   sort      per pass: tile digit counts (k_bs_up from the keys in the first
+//             pass, k_bs_up_dig from the one-byte digits the previous
+//             scatter wrote in later ones), a scan over tiles, k_bs_down
+//             (rank, stage digit-sorted in LDS, write digit runs and the
+//             next pass's digit bytes)
 //   k_seg     one wavefront per window of 64 sorted items: a segmented scan
 //             sums the packed byte increments of each bucket run; the run's
 //             last lane owns it.  Linked bucket: add to the high row.  Unlinked
@@ -252,6 +255,47 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
         hist[(size_t)blockIdx.x * nbins + t] = (uint16_t)s_h[t]; // tile-major: one contiguous row per tile
 }
 
+// Tile counts of a later pass, from the digit bytes the previous scatter wrote
+// (one byte per item read instead of the 8-byte item).  Thread t counts the
+// tile's items t * OS_IPL .. t * OS_IPL + OS_IPL - 1, with one LDS add per run
+// of equal digits (after the first pass the items of one cell are adjacent).
+// The digit buffer holds whole tiles, so every load is in bounds; bytes past n
+// are not counted.
+static_assert(OS_IPL % 8 == 0, "digit bytes are read 8 at a time");
+__global__ __launch_bounds__(OS_THREADS) void k_bs_up_dig(const uint8_t *__restrict__ dig, uint32_t n,
+                                                          uint32_t nbins, uint16_t *__restrict__ hist)
+{
+    __shared__ uint32_t s_h[OS_BINS];
+    const uint32_t t = threadIdx.x;
+    const uint32_t i0 = blockIdx.x * OS_TILE + t * OS_IPL;
+    uint64_t w[OS_IPL / 8];
+#pragma unroll
+    for (int k = 0; k < OS_IPL / 8; ++k)
+        w[k] = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(dig + i0) + k);
+    if (t < OS_BINS)
+        s_h[t] = 0;
+    __syncthreads();
+    const uint32_t nv = i0 < n ? (n - i0 < OS_IPL ? n - i0 : OS_IPL) : 0u; // counted items of this thread
+    uint32_t cur = 0, run = 0;
+#pragma unroll
+    for (int j = 0; j < OS_IPL; ++j) {
+        const uint32_t d = (uint32_t)(w[j >> 3] >> (8 * (j & 7))) & 0xffu;
+        if ((uint32_t)j < nv) {
+            if (run && d != cur) {
+                atomicAdd(&s_h[cur], run);
+                run = 0;
+            }
+            cur = d;
+            ++run;
+        }
+    }
+    if (run)
+        atomicAdd(&s_h[cur], run);
+    __syncthreads();
+    if (t < nbins)
+        hist[(size_t)blockIdx.x * nbins + t] = (uint16_t)s_h[t];
+}
+
 // Two-level exclusive scan over tiles of the tile-major digit counts.
 // Level 1: each block takes SCAN_TPB consecutive tiles, thread d = digit d
 // (reads and writes are contiguous rows), and leaves group-local prefixes plus
@@ -371,7 +415,8 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const void *__restrict__ in, uint32_t n, uint64_t base, qs_sdesc d0, const qs_sdesc *__restrict__ D,
     const uint64_t *__restrict__ offs, uint32_t nsk, qs_pack pk, int shift, uint32_t dmask,
     uint32_t nbins, uint32_t ntiles, const uint16_t *__restrict__ hist, const uint32_t *__restrict__ gsum,
-    const uint32_t *__restrict__ totals, uint64_t *__restrict__ out, qs_dev_state *__restrict__ st)
+    const uint32_t *__restrict__ totals, uint64_t *__restrict__ out, uint8_t *__restrict__ dig, int nshift,
+    qs_dev_state *__restrict__ st)
 {
     __shared__ uint64_t s_item[OS_TILE];
     __shared__ uint32_t s_wh[OS_WAVES][OS_BINS];
@@ -476,10 +521,13 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const uint32_t tn = (n - tb) < OS_TILE ? (n - tb) : OS_TILE;
     for (uint32_t q = t; q < tn; q += OS_THREADS) {
         const uint64_t it = s_item[q];
-        const uint32_t dst = s_goff[(uint32_t)(it_cell(it, pk) >> shift) & dmask] + q;
-        if (dst < n) // the tile counts and the ranking agree; checked, never trusted
+        const uint64_t c = it_cell(it, pk);
+        const uint32_t dst = s_goff[(uint32_t)(c >> shift) & dmask] + q;
+        if (dst < n) { // the tile counts and the ranking agree; checked, never trusted
             out[dst] = it;
-        else
+            if (nshift >= 0) // the next pass's digit, for its tile counts
+                dig[dst] = (uint8_t)((uint32_t)(c >> nshift) & dmask);
+        } else
             atomicOr(&st->err, QS_EF_SORT);
     }
 }
@@ -1625,6 +1673,8 @@ void qs_free_scratch(qs_sketch *s)
         s->d_tk[k] = s->d_tv[k] = nullptr;
     }
     hipFree(s->d_hist);
+    hipFree(s->d_dig);
+    s->d_dig = nullptr;
     hipFree(s->d_ord);
     hipFree(s->d_long);
     hipFree(s->d_trig);
@@ -1651,6 +1701,7 @@ int qs_ensure_scratch(qs_sketch *s, uint64_t chunk)
             return QS_ERR_NOMEM;
     }
     if (hipMalloc(&s->d_hist, sizeof(uint32_t) * qs_scratch_hist_words(chunk)) != hipSuccess ||
+        hipMalloc(&s->d_dig, (chunk + OS_TILE - 1) / OS_TILE * OS_TILE) != hipSuccess ||
         hipMalloc(&s->d_ord, sizeof(uint32_t) * chunk) != hipSuccess ||
         hipMalloc(&s->d_long, sizeof(uint32_t) * (chunk / 8 + 64)) != hipSuccess ||
         hipMalloc(&s->d_trig, sizeof(qs_trig) * tcap) != hipSuccess)
@@ -1683,6 +1734,7 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
     int out = 0;
     for (int ps = 0; ps < passes; ++ps) {
         const int shift = dbits * ps;
+        const int nshift = ps + 1 < passes ? shift + dbits : -1; // the scatter writes the next digit
         qs_evpair e0, e1, e2;
         if (bc.timed)
             qs_tim_begin(s, e0, QS_TK_SORT_UP, cn, stream);
@@ -1691,8 +1743,7 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
                                tcount);
         else
-            hipLaunchKernelGGL((k_bs_up<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
-                               base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
+            hipLaunchKernelGGL(k_bs_up_dig, dim3(ntiles), dim3(OS_THREADS), 0, stream, s->d_dig, cn, nbins,
                                tcount);
         if (bc.timed) {
             qs_tim_end(s, e0, stream);
@@ -1707,11 +1758,11 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
         if (ps == 0)
             hipLaunchKernelGGL((k_bs_down<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               tcount, gsum, totals, s->d_it[out], s->d_st);
+                               tcount, gsum, totals, s->d_it[out], s->d_dig, nshift, s->d_st);
         else
             hipLaunchKernelGGL((k_bs_down<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               tcount, gsum, totals, s->d_it[out], s->d_st);
+                               tcount, gsum, totals, s->d_it[out], s->d_dig, nshift, s->d_st);
         if (bc.timed)
             qs_tim_end(s, e2, stream);
         in = s->d_it[out];

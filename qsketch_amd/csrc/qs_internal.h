@@ -285,6 +285,7 @@ struct qs_sketch {
     uint64_t chunk;         // capacity (keys) of the chunk scratch
     uint64_t *d_it[2];      // packed chunk items sorted by cell (double buffer)
     uint32_t *d_hist;       // radix histograms (both sorts)
+    uint8_t *d_dig;         // the next radix pass's digit of each sorted item (one byte per item)
     uint32_t *d_ord;        // heads of segments for the in-order replay
     uint32_t *d_long;       // heads of long segments
     qs_trig *d_trig;
