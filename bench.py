@@ -159,12 +159,13 @@ def main():
     default = all(getattr(a, k) == v for k, v in DEFAULT_WORKLOAD.items())
     traffic = round(INSERT_TRAFFIC_BYTES_PER_KEY * keys_per_launch) if default else None
     # the dominant kernel on its own: the radix scatter moves one 8-byte item
-    # in and out per key and pass (the first pass reads the 4-byte key)
+    # in and out per key and pass (the first pass reads the 4-byte key), and
+    # every pass but the last writes the next pass's one-byte digit
     sd = kern.get("sort_down")
     dom_obj = None
     if sd:
         passes_per_chunk = sd["launches"] / chunks
-        item_bytes = (4 + 8 + (passes_per_chunk - 1) * 16) / passes_per_chunk
+        item_bytes = (4 + 8 + (passes_per_chunk - 1) * 16 + (passes_per_chunk - 1)) / passes_per_chunk
         sd_avg = sd["ms"] / sd["launches"]
         dom_obj = {"kernel": "k_bs_down (radix scatter)", "avg_launch_ms": round(sd_avg, 4),
                    "algorithmic_bytes_per_launch": round(item_bytes * keys_per_launch),
@@ -192,8 +193,8 @@ def main():
         "query_gkeys_s": round(tim["query_keys"] / (q_ms * 1e-3) / 1e9, 3) if q_ms > 0 else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": "insert path: per radix pass k_bs_up, k_hist_scan_local, k_hist_scan_groups, "
-                      "k_bs_down; then k_segw, k_ordered, k_trig_keys + pair sort, k_assign, k_tail, "
+            "kernel": "insert path: per radix pass k_bs_up (first pass) or k_bs_up_dig, "
+                      "k_hist_scan_local, k_hist_scan_groups, k_bs_down; then k_segw, k_ordered, k_trig_keys + pair sort, k_assign, k_tail, "
                       "k_clear_long, k_next, k_reset",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

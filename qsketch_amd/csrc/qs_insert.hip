@@ -256,44 +256,60 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
 }
 
 // Tile counts of a later pass, from the digit bytes the previous scatter wrote
-// (one byte per item read instead of the 8-byte item).  Thread t counts the
-// tile's items t * OS_IPL .. t * OS_IPL + OS_IPL - 1, with one LDS add per run
-// of equal digits (after the first pass the items of one cell are adjacent).
-// The digit buffer holds whole tiles, so every load is in bounds; bytes past n
-// are not counted.
+// (one byte per item read instead of the 8-byte item).  A block counts
+// UP_TILES tiles, each into its own LDS row; in each, thread t counts the
+// items t * OS_IPL .. t * OS_IPL + OS_IPL - 1, with one LDS add per run of
+// equal digits (after the first pass the items of one cell are adjacent).
+// The digit buffer holds whole tiles and loads of tiles past the last read the
+// last one, so every load is in bounds; items past n are not counted.
+#define UP_TILES 4u
 static_assert(OS_IPL % 8 == 0, "digit bytes are read 8 at a time");
 __global__ __launch_bounds__(OS_THREADS) void k_bs_up_dig(const uint8_t *__restrict__ dig, uint32_t n,
-                                                          uint32_t nbins, uint16_t *__restrict__ hist)
+                                                          uint32_t ntiles, uint32_t nbins,
+                                                          uint16_t *__restrict__ hist)
 {
-    __shared__ uint32_t s_h[OS_BINS];
+    __shared__ uint32_t s_h[UP_TILES][OS_BINS];
     const uint32_t t = threadIdx.x;
-    const uint32_t i0 = blockIdx.x * OS_TILE + t * OS_IPL;
-    uint64_t w[OS_IPL / 8];
+    const uint32_t tile0 = blockIdx.x * UP_TILES;
+    uint64_t w[UP_TILES][OS_IPL / 8];
 #pragma unroll
-    for (int k = 0; k < OS_IPL / 8; ++k)
-        w[k] = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(dig + i0) + k);
-    if (t < OS_BINS)
-        s_h[t] = 0;
-    __syncthreads();
-    const uint32_t nv = i0 < n ? (n - i0 < OS_IPL ? n - i0 : OS_IPL) : 0u; // counted items of this thread
-    uint32_t cur = 0, run = 0;
+    for (uint32_t k = 0; k < UP_TILES; ++k) {
+        const uint32_t tl = tile0 + k < ntiles ? tile0 + k : ntiles - 1u;
+        const uint8_t *src = dig + (size_t)tl * OS_TILE + t * OS_IPL;
 #pragma unroll
-    for (int j = 0; j < OS_IPL; ++j) {
-        const uint32_t d = (uint32_t)(w[j >> 3] >> (8 * (j & 7))) & 0xffu;
-        if ((uint32_t)j < nv) {
-            if (run && d != cur) {
-                atomicAdd(&s_h[cur], run);
-                run = 0;
-            }
-            cur = d;
-            ++run;
-        }
+        for (int j = 0; j < OS_IPL / 8; ++j)
+            w[k][j] = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(src) + j);
     }
-    if (run)
-        atomicAdd(&s_h[cur], run);
+    if (t < OS_BINS)
+#pragma unroll
+        for (uint32_t k = 0; k < UP_TILES; ++k)
+            s_h[k][t] = 0;
     __syncthreads();
-    if (t < nbins)
-        hist[(size_t)blockIdx.x * nbins + t] = (uint16_t)s_h[t];
+#pragma unroll
+    for (uint32_t k = 0; k < UP_TILES; ++k) {
+        const uint64_t i0 = (uint64_t)(tile0 + k) * OS_TILE + t * OS_IPL;
+        const uint32_t nv = i0 < n ? (n - i0 < OS_IPL ? (uint32_t)(n - i0) : OS_IPL) : 0u; // counted items
+        uint32_t cur = 0, run = 0;
+#pragma unroll
+        for (int j = 0; j < OS_IPL; ++j) {
+            const uint32_t d = (uint32_t)(w[k][j >> 3] >> (8 * (j & 7))) & 0xffu;
+            if ((uint32_t)j < nv) {
+                if (run && d != cur) {
+                    atomicAdd(&s_h[k][cur], run);
+                    run = 0;
+                }
+                cur = d;
+                ++run;
+            }
+        }
+        if (run)
+            atomicAdd(&s_h[k][cur], run);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < UP_TILES; ++k)
+        if (tile0 + k < ntiles && t < nbins)
+            hist[(size_t)(tile0 + k) * nbins + t] = (uint16_t)s_h[k][t];
 }
 
 // Two-level exclusive scan over tiles of the tile-major digit counts.
@@ -450,6 +466,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
     const uint32_t dbase = block_excl_scan(tot, s_red);
 
     const uint64_t lt = (1ull << lane) - 1ull;
+    const int dbits = __builtin_popcount(dmask);
     uint64_t item[OS_IPL];
     uint32_t dr[OS_IPL]; // digit | wave-local position << 8 | valid << 31
     {
@@ -477,9 +494,11 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_down(
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; ++bit) {
-            const bool on = (d >> bit) & 1u;
-            const uint64_t bb = __ballot(on);
-            m &= on ? bb : ~bb;
+            if (bit < dbits) { // uniform: the digit width is a launch argument
+                const bool on = (d >> bit) & 1u;
+                const uint64_t bb = __ballot(on);
+                m &= on ? bb : ~bb;
+            }
         }
         const uint32_t rank = (uint32_t)__popcll(m & lt);
         const int leader = valid ? (int)__builtin_ctzll(m) : 0;
@@ -1743,8 +1762,8 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
                                tcount);
         else
-            hipLaunchKernelGGL(k_bs_up_dig, dim3(ntiles), dim3(OS_THREADS), 0, stream, s->d_dig, cn, nbins,
-                               tcount);
+            hipLaunchKernelGGL(k_bs_up_dig, dim3((ntiles + UP_TILES - 1) / UP_TILES), dim3(OS_THREADS), 0, stream,
+                               s->d_dig, cn, ntiles, nbins, tcount);
         if (bc.timed) {
             qs_tim_end(s, e0, stream);
             qs_tim_begin(s, e1, QS_TK_SORT_SCAN, cn, stream);
