@@ -195,7 +195,9 @@ __device__ static inline RawT<KB, FIRST> load_raw(const void *in, uint32_t i, ui
     return __builtin_nontemporal_load(reinterpret_cast<const RawT<KB, FIRST> *>(in) + ic);
 }
 
-template <int KB, bool FIRST, bool MULTI>
+// Tile counts of the first pass, from the keys: each item is built on the way
+// (the scatter builds it again rather than reading it back)
+template <int KB, bool MULTI>
 __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ in, uint32_t n, uint64_t base,
                                                       qs_sdesc d0, const qs_sdesc *__restrict__ D,
                                                       const uint64_t *__restrict__ offs, uint32_t nsk,
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
         s_h[t] = 0;
     const uint32_t tb = blockIdx.x * OS_TILE;
     sid_window win{s_off, 0u, 0u};
-    if (FIRST && MULTI) {
+    if (MULTI) {
         const uint32_t te = (n - tb) < OS_TILE ? n : tb + OS_TILE;
         load_sid_window(offs, nsk, base + tb, base + te - 1u, s_off, s_w);
         win.sid0 = s_w[0];
@@ -221,21 +223,17 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
     const uint32_t lane = t & 63u;
     // every load first, from clamped positions: a load inside a per-item branch
     // is issued and waited for one at a time
-    RawT<KB, FIRST> raw[OS_IPL];
+    KeyT<KB> raw[OS_IPL];
 #pragma unroll
     for (int r = 0; r < OS_IPL; ++r)
-        raw[r] = load_raw<KB, FIRST>(in, tb + r * OS_THREADS + t, n);
+        raw[r] = load_raw<KB, true>(in, tb + r * OS_THREADS + t, n);
 #pragma unroll
     for (int r = 0; r < OS_IPL; ++r) {
         const uint32_t i = tb + r * OS_THREADS + t; // a wavefront's lanes hold consecutive items
         const bool valid = i < n;
         uint32_t d = 0;
         if (valid) {
-            uint64_t c;
-            if (FIRST)
-                c = it_cell(make_item<KB, MULTI>((KeyT<KB>)raw[r], i, base, d0, D, offs, nsk, pk, win), pk);
-            else
-                c = it_cell((uint64_t)raw[r], pk);
+            const uint64_t c = it_cell(make_item<KB, MULTI>(raw[r], i, base, d0, D, offs, nsk, pk, win), pk);
             d = (uint32_t)(c >> shift) & dmask;
         }
         // one LDS add per run of equal digits on adjacent lanes (a Zipf-hot
@@ -1758,7 +1756,7 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
         if (bc.timed)
             qs_tim_begin(s, e0, QS_TK_SORT_UP, cn, stream);
         if (ps == 0)
-            hipLaunchKernelGGL((k_bs_up<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
+            hipLaunchKernelGGL((k_bs_up<KB, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
                                tcount);
         else
