@@ -42,8 +42,9 @@ def parse():
     ap.add_argument("--wlf", type=float, default=0.5, help="work-load factor of the sizing rule")
     ap.add_argument("--n-high", type=int, default=1 << 18)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-log2", type=int, default=24,
-                    help="log2 of the stream prefix the CPU baseline inserts")
+    ap.add_argument("--cpu-sample-log2", type=int, default=27,
+                    help="log2 of the stream prefix the CPU baseline inserts (2^27: about 20 s of "
+                         "thread time on 16 host threads)")
     ap.add_argument("--workload", choices=["plain", "multi", "learned"], default="plain",
                     help="plain: one sketch (default); multi: 4096 independent sketches in one batch; "
                          "learned: MLP-routed heavy hitters + sketch vs plain sketch at equal memory")
@@ -82,7 +83,8 @@ def cpu_baseline(keys_dev, n_low, n_high, sample_log2):
         "kind": "port",
         "sample": f"first 2^{sample_log2} keys of the same stream inserted ({t1 - t0:.2f} s, "
                   f"{n / (t1 - t0) / 1e6:.1f} Mkeys/s), then its {uniq.size} distinct keys queried "
-                  f"({t2 - t1:.2f} s); bucket-range partitioned oracle on {threads} threads",
+                  f"({t2 - t1:.2f} s); bucket-range partitioned oracle on {threads} threads, "
+                  f"{(t2 - t0) * threads:.1f} thread-seconds",
     }
 
 
