@@ -23,6 +23,12 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
+# Per-source additions.  The learned forward reads its MFMA accumulators with
+# VALU instructions (ReLU and BF16 packing), which take no AGPR
+# operands: accumulators in VGPRs save one v_accvgpr_read per value.
+SOURCE_FLAGS = {
+    "qs_learned.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
 
 
 def _run(cmd):
@@ -43,7 +49,7 @@ def build(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", path, "-o", obj]
+        cmd = [HIPCC, *FLAGS, *SOURCE_FLAGS.get(src, []), "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)

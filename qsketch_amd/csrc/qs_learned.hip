@@ -311,7 +311,9 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const KeyT<KB> *__restrict__ ke
 // 2 s + h.  Layer 2 sums over the units in the order the accumulators hold
 // them: K step s takes registers 8 (s % 2) .. 8 (s % 2) + 7 of row tile s / 2,
 // so slot i of half h is unit 32 (s / 2) + 8 (2 (s % 2) + i / 4) + 4 h + i % 4
-// (every unit once), and W2's columns are laid out in that order.
+// (every unit once), and W2's columns are laid out in that order.  The output
+// layer is one more row tile in the same K order: W3 is its row 0 (the other
+// rows are zero), so the score of key l is register 0 of lane l < 32.
 // ---------------------------------------------------------------------------
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
@@ -324,14 +326,40 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // then hold only accumulators and activations, for several waves per SIMD.
 struct mlp_lds {
     bf16x8 a[2][2][4][64];
-    bf16x4 nib[16]; // layer-1 B operand of a key nibble: bit i -> 1.0 or 0.0 in BF16
-    float b1[64], b2[64], w3[64], b3;
+    bf16x8 a3[4][64]; // output layer: W3 in row 0 of the A operand, other rows 0
+    bf16x4 nib[16];   // layer-1 B operand of a key nibble: bit i -> 1.0 or 0.0 in BF16
+    float b1[64], b2[64], b3;
 };
 
-// max(x, 0) on the bits: negative floats are negative integers (no NaN here)
-__device__ static inline float relu_bits(float x)
+// BF16(max(lo, 0)), BF16(max(hi, 0)) packed: one conversion of the pair, then
+// max(., 0) on the two 16-bit halves (rounding keeps the sign, and negative
+// BF16 values and -0 are negative 16-bit integers: the same bits as rounding
+// max(x, 0))
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+__device__ static inline uint32_t relu_bf16x2(float lo, float hi)
 {
-    return __int_as_float(max(__float_as_int(x), 0));
+    const bf16x2v b = __builtin_convertvector((f32x2v){lo, hi}, bf16x2v); // v_cvt_pk_bf16_f32
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, b), (s16x2)0));
+}
+
+// a layer's activations as the next layer's B operands: K step s takes
+// registers 8 (s % 2) .. 8 (s % 2) + 7 of row tile s / 2
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+__device__ static inline void acts_to_b(const float *acc0, const float *acc1, bf16x8 hb[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const float *a = s < 2 ? acc0 : acc1;
+        u32x4v w;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int r = 8 * (s % 2) + 2 * p;
+            w[p] = relu_bf16x2(a[r], a[r + 1]);
+        }
+        hb[s] = __builtin_bit_cast(bf16x8, w);
+    }
 }
 
 // input column of K slot i, half h, K step s of layer 0 / 1
@@ -352,6 +380,14 @@ __device__ static inline void load_mlp(mlp_lds &S, const uint16_t *__restrict__ 
             v[i] = (short)w[kcol(layer, s, h, (uint32_t)i)];
         S.a[layer][t][s][l] = v;
     }
+    for (uint32_t e = threadIdx.x; e < 4u * 64u; e += blockDim.x) {
+        const uint32_t l = e & 63u, s = e >> 6, h = l >> 5, c = l & 31u;
+        bf16x8 v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            v[i] = c == 0 ? (short)pb[OFF_W3 + kcol(1, s, h, (uint32_t)i)] : (short)0;
+        S.a3[s][l] = v;
+    }
     for (uint32_t e = threadIdx.x; e < 16u; e += blockDim.x) {
         bf16x4 v;
 #pragma unroll
@@ -362,7 +398,6 @@ __device__ static inline void load_mlp(mlp_lds &S, const uint16_t *__restrict__ 
     for (uint32_t e = threadIdx.x; e < 64u; e += blockDim.x) {
         S.b1[e] = p[OFF_B1 + e];
         S.b2[e] = p[OFF_B2 + e];
-        S.w3[e] = bf2f(pb[OFF_W3 + e]);
     }
     if (threadIdx.x == 0)
         S.b3 = p[OFF_B3];
@@ -381,7 +416,8 @@ __device__ static inline f32x16 mfma16(const bf16x8 &a, const bf16x8 &b, const f
                                                    0, 0, 0);
 }
 
-// score of key x (lane l scores key l % 32; both half-waves return it)
+// score of key x: lane l < 32 returns the score of its key (lanes 32..63
+// return another value of the output tile)
 __device__ static inline float mlp_mfma(const mlp_lds &S, uint64_t x)
 {
     const uint32_t lane = threadIdx.x & 63u, h = lane >> 5;
@@ -404,13 +440,15 @@ __device__ static inline float mlp_mfma(const mlp_lds &S, uint64_t x)
         acc1 = mfma16(S.a[0][1][s][lane], b, acc1);
     }
     bf16x8 hb[4];
+    {
+        float a0[16], a1[16];
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int r = 8 * (s % 2) + i;
-            hb[s][i] = (short)f2bf(relu_bits(s < 2 ? acc0[r] : acc1[r]));
+        for (int r = 0; r < 16; ++r) {
+            a0[r] = acc0[r];
+            a1[r] = acc1[r];
         }
+        acts_to_b(a0, a1, hb);
+    }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         acc0[r] = S.b2[drow(0, r, h)];
@@ -421,14 +459,20 @@ __device__ static inline float mlp_mfma(const mlp_lds &S, uint64_t x)
         acc0 = mfma16(S.a[1][0][s][lane], hb[s], acc0);
         acc1 = mfma16(S.a[1][1][s][lane], hb[s], acc1);
     }
-    float y = 0.f;
+    {
+        float a0[16], a1[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        y = fmaf(S.w3[drow(0, r, h)], bf2f(f2bf(relu_bits(acc0[r]))), y);
-        y = fmaf(S.w3[drow(1, r, h)], bf2f(f2bf(relu_bits(acc1[r]))), y);
+        for (int r = 0; r < 16; ++r) {
+            a0[r] = acc0[r];
+            a1[r] = acc1[r];
+        }
+        acts_to_b(a0, a1, hb);
     }
-    y += __shfl_xor(y, 32, 64);
-    return y + S.b3;
+    f32x16 acc3 = {};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        acc3 = mfma16(S.a3[s][lane], hb[s], acc3);
+    return acc3[0] + S.b3;
 }
 
 template <int KB>
