@@ -315,7 +315,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up_dig(const uint8_t *__restr
 // (reads and writes are contiguous rows), and leaves group-local prefixes plus
 // the group's sums.  Level 2: one block per digit scans that digit's group
 // sums and writes its total.  The scatter adds the two levels.
-#define SCAN_TPB 16u
+#define SCAN_TPB 32u
 // The tile counts and their group-local prefixes are 16-bit: a count is at
 // most OS_TILE and a prefix at most (SCAN_TPB - 1) * OS_TILE.
 static_assert(SCAN_TPB * OS_TILE <= 65536u, "16-bit tile counts");
