@@ -257,7 +257,8 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up(const void *__restrict__ i
 // (one byte per item read instead of the 8-byte item).  A block counts
 // UP_TILES tiles, each into its own LDS row; in each, thread t counts the
 // items t * OS_IPL .. t * OS_IPL + OS_IPL - 1, with one LDS add per run of
-// equal digits (after the first pass the items of one cell are adjacent).
+// equal digits (after the first pass the items of one cell are adjacent);
+// runs of threads that hold a single digit are added once per run of lanes.
 // The digit buffer holds whole tiles and loads of tiles past the last read the
 // last one, so every load is in bounds; items past n are not counted.
 #define UP_TILES 4u
@@ -283,25 +284,45 @@ __global__ __launch_bounds__(OS_THREADS) void k_bs_up_dig(const uint8_t *__restr
         for (uint32_t k = 0; k < UP_TILES; ++k)
             s_h[k][t] = 0;
     __syncthreads();
+    const uint32_t lane = t & 63u;
 #pragma unroll
     for (uint32_t k = 0; k < UP_TILES; ++k) {
         const uint64_t i0 = (uint64_t)(tile0 + k) * OS_TILE + t * OS_IPL;
         const uint32_t nv = i0 < n ? (n - i0 < OS_IPL ? (uint32_t)(n - i0) : OS_IPL) : 0u; // counted items
-        uint32_t cur = 0, run = 0;
+        // A thread whose items all carry one digit is counted together with
+        // its neighbours: one add per run of such lanes (a Zipf-hot cell fills
+        // whole tiles, and one add per lane to one bin would serialise).  The
+        // ballots and the shuffle are taken by the whole wavefront.
+        const uint32_t d0 = (uint32_t)w[k][0] & 0xffu;
+        bool whole = nv == OS_IPL;
 #pragma unroll
-        for (int j = 0; j < OS_IPL; ++j) {
-            const uint32_t d = (uint32_t)(w[k][j >> 3] >> (8 * (j & 7))) & 0xffu;
-            if ((uint32_t)j < nv) {
-                if (run && d != cur) {
-                    atomicAdd(&s_h[k][cur], run);
-                    run = 0;
+        for (int j = 0; j < OS_IPL / 8; ++j)
+            whole = whole && w[k][j] == 0x0101010101010101ull * d0;
+        const uint64_t wm = __ballot(whole);
+        const uint32_t dprev = (uint32_t)__shfl_up((int)d0, 1, 64);
+        const bool head = whole && !(lane > 0u && ((wm >> (lane - 1u)) & 1ull) && dprev == d0);
+        const uint64_t heads = __ballot(head);
+        const uint64_t later = (heads | ~wm) & ~((2ull << lane) - 1ull); // lanes that end my run
+        const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : 64u;
+        if (head)
+            atomicAdd(&s_h[k][d0], (end - lane) * OS_IPL);
+        if (!whole) {
+            uint32_t cur = 0, run = 0;
+#pragma unroll
+            for (int j = 0; j < OS_IPL; ++j) {
+                const uint32_t d = (uint32_t)(w[k][j >> 3] >> (8 * (j & 7))) & 0xffu;
+                if ((uint32_t)j < nv) {
+                    if (run && d != cur) {
+                        atomicAdd(&s_h[k][cur], run);
+                        run = 0;
+                    }
+                    cur = d;
+                    ++run;
                 }
-                cur = d;
-                ++run;
             }
+            if (run)
+                atomicAdd(&s_h[k][cur], run);
         }
-        if (run)
-            atomicAdd(&s_h[k][cur], run);
     }
     __syncthreads();
 #pragma unroll
