@@ -111,6 +111,23 @@ int qs_query_async(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_count
                    void *stream);
 int qs_sync(qs_sketch *s);
 
+/* Allocation-free insert.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * qs_insert_async(s, d_keys, n, stream) and stream-ordered like it (errors
+ * surface at the next blocking call or qs_sync), but the insert scratch lives
+ * in one static device array of 33 MiB per GPU, held from library load until
+ * the process ends, so the call allocates and frees nothing.  Limits: a plain
+ * sketch (a set of one), n <= QS_NOALLOC_MAX_KEYS, and at most
+ * QS_NOALLOC_MAX_ALLOCS possible high-bucket allocations, i.e.
+ * min(n_high, n) <= QS_NOALLOC_MAX_ALLOCS; otherwise QS_ERR_PARAM.  The
+ * current device must be the sketch's.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (several streams, several
+ * threads) race and corrupt each other's scratch; order them on one stream. */
+#define QS_NOALLOC_MAX_KEYS   (1u << 20)
+#define QS_NOALLOC_MAX_ALLOCS (1u << 16)
+int qs_insert_noalloc(qs_sketch *s, const void *d_keys, size_t n, void *stream);
+
 /* ---- state import / export ---------------------------------------------- */
 /* Fills the reference layout: low  = uint32[n_low*8]  (word 7 = link),
  * high = uint32[n_high*32], masks = uint32[n_masks] (little-endian bit j = byte

@@ -30,6 +30,8 @@ QS_ERR_INTERNAL = -8
 QS_LITERAL = 0
 QS_PAPER = 1
 MLP_PARAMS = 8385
+NOALLOC_MAX_KEYS = 1 << 20     # include/qsketch.h QS_NOALLOC_MAX_KEYS
+NOALLOC_MAX_ALLOCS = 1 << 16   # include/qsketch.h QS_NOALLOC_MAX_ALLOCS
 
 # lib/default.cu:39-54 and src/main.cu:390-391
 NEXT_LEVEL_ID_START = 1024
@@ -100,6 +102,7 @@ _SIGS = {
     "qs_insert_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "qs_query_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     "qs_sync": (C.c_int, [C.c_void_p]),
+    "qs_insert_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "qs_export_state": (C.c_int, [C.c_void_p] + [C.c_void_p] * 5),
     "qs_import_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "qs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(qs_stats)]),
@@ -331,6 +334,13 @@ class Sketch_GPU_Sub_Warp_Level:
     def search_async(self, keys, n: int, counts, stream: int = 0) -> None:
         check(_lib().qs_query_async(self._h, _ptr(keys), int(n), _ptr(counts), C.c_void_p(stream)),
               "qs_query_async")
+
+    def insert_noalloc(self, keys, n: int | None = None, stream: int = 0) -> None:
+        """Stream-ordered insert whose scratch is the library's static device
+        array (no allocation): n <= NOALLOC_MAX_KEYS, min(n_high, n) <=
+        NOALLOC_MAX_ALLOCS."""
+        n = keys.numel() if n is None else int(n)
+        check(_lib().qs_insert_noalloc(self._h, _ptr(keys), n, C.c_void_p(stream)), "qs_insert_noalloc")
 
     def sync(self) -> None:
         check(_lib().qs_sync(self._h), "qs_sync")

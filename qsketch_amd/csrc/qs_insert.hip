@@ -1703,48 +1703,79 @@ uint64_t qs_scratch_hist_words(uint64_t chunk)
 
 void qs_free_scratch(qs_sketch *s)
 {
+    qs_scratch &sc = s->sc;
     for (int k = 0; k < 2; ++k) {
-        hipFree(s->d_it[k]);
-        hipFree(s->d_tk[k]);
-        hipFree(s->d_tv[k]);
-        s->d_it[k] = nullptr;
-        s->d_tk[k] = s->d_tv[k] = nullptr;
+        hipFree(sc.it[k]);
+        hipFree(sc.tk[k]);
+        hipFree(sc.tv[k]);
     }
-    hipFree(s->d_hist);
-    hipFree(s->d_dig);
-    s->d_dig = nullptr;
-    hipFree(s->d_ord);
-    hipFree(s->d_long);
-    hipFree(s->d_trig);
-    s->d_hist = s->d_ord = s->d_long = nullptr;
-    s->d_trig = nullptr;
-    s->chunk = 0;
+    hipFree(sc.hist);
+    hipFree(sc.dig);
+    hipFree(sc.ord);
+    hipFree(sc.lng);
+    hipFree(sc.trig);
+    memset(&sc, 0, sizeof(sc));
+}
+
+uint32_t qs_trig_cap(const qs_sketch *s, uint64_t chunk)
+{
+    uint64_t pool = 0;
+    for (uint32_t i = 0; i < s->nsk; ++i)
+        pool += s->ps[i].n_high;
+    return (uint32_t)(pool < chunk ? pool : chunk);
+}
+
+uint64_t qs_scratch_bytes(uint64_t chunk, uint32_t tcap, qs_scratch *sc, uint8_t *base)
+{
+    // 256-byte aligned pieces, in this order
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        uint8_t *p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~255ull;
+        return p;
+    };
+    qs_scratch t;
+    memset(&t, 0, sizeof(t));
+    for (int k = 0; k < 2; ++k)
+        t.it[k] = (uint64_t *)take(sizeof(uint64_t) * chunk);
+    t.hist = (uint32_t *)take(sizeof(uint32_t) * qs_scratch_hist_words(chunk));
+    t.dig = take((chunk + OS_TILE - 1) / OS_TILE * OS_TILE);
+    t.ord = (uint32_t *)take(sizeof(uint32_t) * chunk);
+    t.lng = (uint32_t *)take(sizeof(uint32_t) * (chunk / 8 + 64));
+    t.trig = (qs_trig *)take(sizeof(qs_trig) * tcap);
+    for (int k = 0; k < 2; ++k) {
+        t.tk[k] = (uint32_t *)take(sizeof(uint32_t) * tcap);
+        t.tv[k] = (uint32_t *)take(sizeof(uint32_t) * tcap);
+    }
+    t.chunk = chunk;
+    t.trig_cap = tcap;
+    if (sc)
+        *sc = t;
+    return off;
 }
 
 int qs_ensure_scratch(qs_sketch *s, uint64_t chunk)
 {
     grid_cap(0, 1); // query the CU count now: never while a launch sequence is captured
-    if (s->chunk >= chunk)
+    if (s->sc.chunk >= chunk)
         return QS_OK;
     qs_free_scratch(s);
-    uint64_t pool = 0;
-    for (uint32_t i = 0; i < s->nsk; ++i)
-        pool += s->ps[i].n_high;
-    uint64_t tcap = pool < chunk ? pool : chunk;
-    s->trig_cap = (uint32_t)tcap;
+    const uint32_t tcap = qs_trig_cap(s, chunk);
+    qs_scratch &sc = s->sc;
+    sc.trig_cap = tcap;
     for (int k = 0; k < 2; ++k) {
-        if (hipMalloc(&s->d_it[k], sizeof(uint64_t) * chunk) != hipSuccess ||
-            hipMalloc(&s->d_tk[k], sizeof(uint32_t) * tcap) != hipSuccess ||
-            hipMalloc(&s->d_tv[k], sizeof(uint32_t) * tcap) != hipSuccess)
+        if (hipMalloc(&sc.it[k], sizeof(uint64_t) * chunk) != hipSuccess ||
+            hipMalloc(&sc.tk[k], sizeof(uint32_t) * tcap) != hipSuccess ||
+            hipMalloc(&sc.tv[k], sizeof(uint32_t) * tcap) != hipSuccess)
             return QS_ERR_NOMEM;
     }
-    if (hipMalloc(&s->d_hist, sizeof(uint32_t) * qs_scratch_hist_words(chunk)) != hipSuccess ||
-        hipMalloc(&s->d_dig, (chunk + OS_TILE - 1) / OS_TILE * OS_TILE) != hipSuccess ||
-        hipMalloc(&s->d_ord, sizeof(uint32_t) * chunk) != hipSuccess ||
-        hipMalloc(&s->d_long, sizeof(uint32_t) * (chunk / 8 + 64)) != hipSuccess ||
-        hipMalloc(&s->d_trig, sizeof(qs_trig) * tcap) != hipSuccess)
+    if (hipMalloc(&sc.hist, sizeof(uint32_t) * qs_scratch_hist_words(chunk)) != hipSuccess ||
+        hipMalloc(&sc.dig, (chunk + OS_TILE - 1) / OS_TILE * OS_TILE) != hipSuccess ||
+        hipMalloc(&sc.ord, sizeof(uint32_t) * chunk) != hipSuccess ||
+        hipMalloc(&sc.lng, sizeof(uint32_t) * (chunk / 8 + 64)) != hipSuccess ||
+        hipMalloc(&sc.trig, sizeof(qs_trig) * tcap) != hipSuccess)
         return QS_ERR_NOMEM;
-    s->chunk = chunk;
+    sc.chunk = chunk;
     return QS_OK;
 }
 
@@ -1752,6 +1783,7 @@ int qs_ensure_scratch(qs_sketch *s, uint64_t chunk)
 struct batch_ctx {
     const uint64_t *d_offs; // device batch offsets (batches of several sketches)
     bool timed;             // record timing events (never while capturing)
+    const qs_scratch *sc;   // the scratch this batch runs in
 };
 
 template <int KB, bool MULTI>
@@ -1764,8 +1796,9 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
     const int dbits = (bits + passes - 1) / passes; // <= 8
     const uint32_t nbins = 1u << dbits, dmask = nbins - 1u;
     const uint32_t ngrp = (ntiles + SCAN_TPB - 1) / SCAN_TPB;
-    uint16_t *tcount = reinterpret_cast<uint16_t *>(s->d_hist); // 16-bit tile counts (half the region)
-    uint32_t *gsum = s->d_hist + (size_t)OS_BINS * ntiles;
+    const qs_scratch &sc = *bc.sc;
+    uint16_t *tcount = reinterpret_cast<uint16_t *>(sc.hist); // 16-bit tile counts (half the region)
+    uint32_t *gsum = sc.hist + (size_t)OS_BINS * ntiles;
     uint32_t *totals = gsum + (size_t)OS_BINS * ngrp;
     const qs_sdesc d0 = s->h_desc[0];
     const void *in = keys;
@@ -1782,7 +1815,7 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
                                tcount);
         else
             hipLaunchKernelGGL(k_bs_up_dig, dim3((ntiles + UP_TILES - 1) / UP_TILES), dim3(OS_THREADS), 0, stream,
-                               s->d_dig, cn, ntiles, nbins, tcount);
+                               sc.dig, cn, ntiles, nbins, tcount);
         if (bc.timed) {
             qs_tim_end(s, e0, stream);
             qs_tim_begin(s, e1, QS_TK_SORT_SCAN, cn, stream);
@@ -1796,14 +1829,14 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
         if (ps == 0)
             hipLaunchKernelGGL((k_bs_down<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               tcount, gsum, totals, s->d_it[out], s->d_dig, nshift, s->d_st);
+                               tcount, gsum, totals, sc.it[out], sc.dig, nshift, s->d_st);
         else
             hipLaunchKernelGGL((k_bs_down<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
-                               tcount, gsum, totals, s->d_it[out], s->d_dig, nshift, s->d_st);
+                               tcount, gsum, totals, sc.it[out], sc.dig, nshift, s->d_st);
         if (bc.timed)
             qs_tim_end(s, e2, stream);
-        in = s->d_it[out];
+        in = sc.it[out];
         out ^= 1;
     }
     CK(hipGetLastError());
@@ -1812,22 +1845,22 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
 }
 
 template <bool MULTI, int W, int IPL>
-static void launch_segw(qs_sketch *s, const uint64_t *IT, uint32_t cn, hipStream_t stream)
+static void launch_segw(qs_sketch *s, const qs_scratch &sc, const uint64_t *IT, uint32_t cn, hipStream_t stream)
 {
     const uint64_t groups = ((uint64_t)cn + W * IPL - 1) / (W * IPL);
     hipLaunchKernelGGL((k_segw<MULTI, W, IPL>), dim3(grid_cap((groups * W + SEG_THREADS - 1) / SEG_THREADS, 8)),
                        dim3(SEG_THREADS), 0, stream, IT, cn, s->d_desc, s->h_desc[0].n_masks, s->d_low, s->d_high,
-                       s->d_masks, s->pk, s->d_longbm, s->d_ord, s->d_long, s->d_st);
+                       s->d_masks, s->pk, s->d_longbm, sc.ord, sc.lng, s->d_st);
 }
 
 template <bool MULTI, int W>
-static void launch_segw_ipl(qs_sketch *s, const uint64_t *IT, uint32_t cn, hipStream_t stream)
+static void launch_segw_ipl(qs_sketch *s, const qs_scratch &sc, const uint64_t *IT, uint32_t cn, hipStream_t stream)
 {
     switch (s->seg_ipl) {
-    case 1: launch_segw<MULTI, W, 1>(s, IT, cn, stream); break;
-    case 2: launch_segw<MULTI, W, 2>(s, IT, cn, stream); break;
-    case 4: launch_segw<MULTI, W, 4>(s, IT, cn, stream); break;
-    default: launch_segw<MULTI, W, 8>(s, IT, cn, stream); break;
+    case 1: launch_segw<MULTI, W, 1>(s, sc, IT, cn, stream); break;
+    case 2: launch_segw<MULTI, W, 2>(s, sc, IT, cn, stream); break;
+    case 4: launch_segw<MULTI, W, 4>(s, sc, IT, cn, stream); break;
+    default: launch_segw<MULTI, W, 8>(s, sc, IT, cn, stream); break;
     }
 }
 
@@ -1839,17 +1872,17 @@ bool qs_seg_shape_ok(uint32_t w, uint32_t ipl)
 }
 
 template <bool MULTI>
-static void launch_seg(qs_sketch *s, const uint64_t *IT, uint32_t cn, hipStream_t stream)
+static void launch_seg(qs_sketch *s, const qs_scratch &sc, const uint64_t *IT, uint32_t cn, hipStream_t stream)
 {
     switch (s->seg_w) {
-    case 8: launch_segw_ipl<MULTI, 8>(s, IT, cn, stream); break;
-    case 16: launch_segw_ipl<MULTI, 16>(s, IT, cn, stream); break;
-    case 32: launch_segw_ipl<MULTI, 32>(s, IT, cn, stream); break;
-    case 64: launch_segw_ipl<MULTI, 64>(s, IT, cn, stream); break;
+    case 8: launch_segw_ipl<MULTI, 8>(s, sc, IT, cn, stream); break;
+    case 16: launch_segw_ipl<MULTI, 16>(s, sc, IT, cn, stream); break;
+    case 32: launch_segw_ipl<MULTI, 32>(s, sc, IT, cn, stream); break;
+    case 64: launch_segw_ipl<MULTI, 64>(s, sc, IT, cn, stream); break;
     default:
         hipLaunchKernelGGL(k_seg<MULTI>, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(SEG_THREADS), 0,
                            stream, IT, cn, s->d_desc, s->h_desc[0].n_masks, s->d_low, s->d_high, s->d_masks, s->pk,
-                           s->d_longbm, s->d_ord, s->d_long, s->d_st);
+                           s->d_longbm, sc.ord, sc.lng, s->d_st);
         break;
     }
 }
@@ -1864,32 +1897,34 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
     int rc = sort_chunk<KB, MULTI>(s, keys, cn, base, bc, stream, res);
     if (rc)
         return rc;
-    const uint64_t *IT = s->d_it[res];
+    const qs_scratch &sc = *bc.sc;
+    const uint64_t *IT = sc.it[res];
     const uint32_t nm0 = s->h_desc[0].n_masks;
     qs_evpair es, eo, ea, et;
     if (bc.timed)
         qs_tim_begin(s, es, QS_TK_SEG, cn, stream);
-    launch_seg<MULTI>(s, IT, cn, stream);
+    launch_seg<MULTI>(s, sc, IT, cn, stream);
     if (bc.timed) {
         qs_tim_end(s, es, stream);
         qs_tim_begin(s, eo, QS_TK_ORDERED, cn, stream);
     }
     hipLaunchKernelGGL(k_ordered, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(256), 0, stream, IT, cn,
-                       s->d_ord, s->d_desc, s->d_low, s->d_masks, s->pk, s->d_longbm, s->d_long, s->d_trig,
-                       s->trig_cap, s->nsk == 1 ? s->h_desc[0].n_masks : 0u, s->d_st);
+                       sc.ord, s->d_desc, s->d_low, s->d_masks, s->pk, s->d_longbm, sc.lng, sc.trig,
+                       sc.trig_cap, s->nsk == 1 ? s->h_desc[0].n_masks : 0u, s->d_st);
     if (bc.timed) {
         qs_tim_end(s, eo, stream);
         qs_tim_begin(s, ea, QS_TK_ASSIGN, cn, stream);
     }
-    hipLaunchKernelGGL(k_trig_keys, dim3(grid_cap(((uint64_t)s->trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
-                       s->d_trig, s->d_st, s->trig_cap, s->pk.ib, s->d_tk[0], s->d_tv[0]);
+    hipLaunchKernelGGL(k_trig_keys, dim3(grid_cap(((uint64_t)sc.trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
+                       sc.trig, s->d_st, sc.trig_cap, s->pk.ib, sc.tk[0], sc.tv[0]);
     const int tbits = (int)(s->pk.sb + s->pk.ib);
-    const int tc = qs_sort_pairs_dev(s->d_tk, s->d_tv, s->d_hist, &s->d_st->n_trig, s->trig_cap,
+    uint32_t *tk[2] = {sc.tk[0], sc.tk[1]}, *tv[2] = {sc.tv[0], sc.tv[1]};
+    const int tc = qs_sort_pairs_dev(tk, tv, sc.hist, &s->d_st->n_trig, sc.trig_cap,
                                      tbits > 32 ? 32 : tbits, stream);
     if (tc < 0)
         return QS_ERR_HIP;
-    hipLaunchKernelGGL(k_assign, dim3(grid_cap(((uint64_t)s->trig_cap + 3) / 4, 8)), dim3(256), 0, stream,
-                       s->d_trig, s->d_tk[tc], s->d_tv[tc], s->trig_cap, s->pk.ib, s->d_desc, s->d_low,
+    hipLaunchKernelGGL(k_assign, dim3(grid_cap(((uint64_t)sc.trig_cap + 3) / 4, 8)), dim3(256), 0, stream,
+                       sc.trig, sc.tk[tc], sc.tv[tc], sc.trig_cap, s->pk.ib, s->d_desc, s->d_low,
                        s->d_high, s->d_tail_of, s->d_next, s->d_st);
     if (bc.timed) {
         qs_tim_end(s, ea, stream);
@@ -1900,16 +1935,16 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
     const dim3 tgrid(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8));
     if (s->pk.bb + s->pk.sb <= 31u)
         hipLaunchKernelGGL((k_tail<MULTI, uint32_t>), tgrid, dim3(256), 0, stream, IT, cn, s->d_desc, nm0, s->d_low,
-                           s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, s->d_long, short_runs,
+                           s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, sc.lng, short_runs,
                            s->d_st);
     else
         hipLaunchKernelGGL((k_tail<MULTI, uint64_t>), tgrid, dim3(256), 0, stream, IT, cn, s->d_desc, nm0, s->d_low,
-                           s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, s->d_long, short_runs,
+                           s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, sc.lng, short_runs,
                            s->d_st);
     hipLaunchKernelGGL(k_clear_long, dim3(grid_cap(((uint64_t)cn / 64 + 255) / 256, 4)), dim3(256), 0, stream,
-                       s->d_long, IT, s->d_desc, s->pk, s->d_longbm, s->d_st);
-    hipLaunchKernelGGL(k_next, dim3(grid_cap(((uint64_t)s->trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
-                       s->d_trig, s->d_tk[tc], s->d_tv[tc], s->trig_cap, s->pk.ib, s->d_next, s->d_st);
+                       sc.lng, IT, s->d_desc, s->pk, s->d_longbm, s->d_st);
+    hipLaunchKernelGGL(k_next, dim3(grid_cap(((uint64_t)sc.trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
+                       sc.trig, sc.tk[tc], sc.tv[tc], sc.trig_cap, s->pk.ib, s->d_next, s->d_st);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, s->d_st);
     if (bc.timed)
         qs_tim_end(s, et, stream);
@@ -1942,7 +1977,7 @@ uint64_t qs_chunk_keys(const qs_sketch *s, const uint64_t *offs)
 
 template <int KB, bool MULTI>
 static int insert_impl(qs_sketch *s, const void *d_keys, const uint64_t *offs, const uint64_t *d_offs,
-                       hipStream_t stream, bool blocking, bool capture)
+                       hipStream_t stream, bool blocking, bool capture, const qs_scratch *fixed)
 {
     const KeyT<KB> *keys = (const KeyT<KB> *)d_keys;
     const uint64_t n = offs[s->nsk];
@@ -1957,14 +1992,23 @@ static int insert_impl(qs_sketch *s, const void *d_keys, const uint64_t *offs, c
         return QS_ERR_PARAM;
     const uint64_t cap = qs_chunk_keys(s, offs);
     const uint64_t need = n < cap ? n : cap;
-    if (s->chunk < need && s->graphs > 0)
-        return QS_ERR_STATE; // a captured graph holds the current scratch buffers
-    int rc = qs_ensure_scratch(s, need);
-    if (rc)
-        return rc;
+    int rc = QS_OK;
+    if (fixed) {
+        // caller-provided scratch: it must hold the whole batch as one chunk
+        // and every allocation the chunk can trigger
+        if (n > cap || n > fixed->chunk || qs_trig_cap(s, n) > fixed->trig_cap)
+            return QS_ERR_PARAM;
+    } else {
+        if (s->sc.chunk < need && s->graphs > 0)
+            return QS_ERR_STATE; // a captured graph holds the current scratch buffers
+        rc = qs_ensure_scratch(s, need);
+        if (rc)
+            return rc;
+    }
     batch_ctx bc;
     bc.d_offs = d_offs;
-    bc.timed = s->timing && !capture;
+    bc.timed = s->timing && !capture && !fixed;
+    bc.sc = fixed ? fixed : &s->sc;
     for (uint64_t done = 0; done < n;) {
         const uint64_t cn64 = n - done < cap ? n - done : cap;
         const uint32_t cn = (uint32_t)cn64;
@@ -1994,11 +2038,12 @@ static int insert_impl(qs_sketch *s, const void *d_keys, const uint64_t *offs, c
 }
 
 int qs_insert_set(qs_sketch *s, const void *d_keys, const uint64_t *offs, const uint64_t *d_offs,
-                  hipStream_t stream, bool blocking, bool capture)
+                  hipStream_t stream, bool blocking, bool capture, const qs_scratch *fixed)
 {
     if (s->nsk == 1)
-        return s->key_bytes == 4 ? insert_impl<4, false>(s, d_keys, offs, d_offs, stream, blocking, capture)
-                                 : insert_impl<8, false>(s, d_keys, offs, d_offs, stream, blocking, capture);
-    return s->key_bytes == 4 ? insert_impl<4, true>(s, d_keys, offs, d_offs, stream, blocking, capture)
-                             : insert_impl<8, true>(s, d_keys, offs, d_offs, stream, blocking, capture);
+        return s->key_bytes == 4
+                   ? insert_impl<4, false>(s, d_keys, offs, d_offs, stream, blocking, capture, fixed)
+                   : insert_impl<8, false>(s, d_keys, offs, d_offs, stream, blocking, capture, fixed);
+    return s->key_bytes == 4 ? insert_impl<4, true>(s, d_keys, offs, d_offs, stream, blocking, capture, fixed)
+                             : insert_impl<8, true>(s, d_keys, offs, d_offs, stream, blocking, capture, fixed);
 }

@@ -258,6 +258,23 @@ struct qs_trig {
     uint32_t hc[32];     // high counters accumulated up to tail_start (incl. trigger)
 };
 
+// Device scratch of the insert pipeline for chunks of up to `chunk` keys.  A
+// handle allocates its own (qs_ensure_scratch); the allocation-free insert
+// carves one out of a static device array instead.  Nothing in it needs to
+// start at zero: every buffer is written before it is read within a chunk.
+struct qs_scratch {
+    uint64_t chunk;         // capacity (keys)
+    uint64_t *it[2];        // packed chunk items sorted by cell (double buffer)
+    uint32_t *hist;         // radix histograms (both sorts), qs_scratch_hist_words(chunk) words
+    uint8_t *dig;           // the next radix pass's digit of each sorted item (one byte per item)
+    uint32_t *ord;          // heads of segments for the in-order replay
+    uint32_t *lng;          // heads of long segments
+    qs_trig *trig;          // allocations found by the replay
+    uint32_t trig_cap;      // entries of trig, tk, tv: at most the pool (sum of n_high) and the chunk
+    uint32_t *tk[2];        // trigger sort keys / values
+    uint32_t *tv[2];
+};
+
 // Host-side handle: a set of nsk sketches (nsk == 1 for the plain API).
 struct qs_sketch {
     uint32_t nsk;
@@ -282,16 +299,7 @@ struct qs_sketch {
     uint64_t *d_offs;       // per sketch: start of its keys in the current batch (+ end)
     uint64_t *h_offs;       // pinned staging for d_offs
 
-    uint64_t chunk;         // capacity (keys) of the chunk scratch
-    uint64_t *d_it[2];      // packed chunk items sorted by cell (double buffer)
-    uint32_t *d_hist;       // radix histograms (both sorts)
-    uint8_t *d_dig;         // the next radix pass's digit of each sorted item (one byte per item)
-    uint32_t *d_ord;        // heads of segments for the in-order replay
-    uint32_t *d_long;       // heads of long segments
-    qs_trig *d_trig;
-    uint32_t trig_cap;
-    uint32_t *d_tk[2];      // trigger sort keys / values
-    uint32_t *d_tv[2];
+    qs_scratch sc;          // the insert scratch this handle allocates (qs_ensure_scratch)
     qs_dev_state *h_st;     // pinned host mirror
 
     int timing;             // record events around kernel launches
@@ -326,9 +334,16 @@ void qs_scan_digits(uint32_t *hist, uint32_t ntiles, uint32_t nbins, uint32_t *t
 // d_offs: device copy of offs (read by the kernels when nsk > 1); may be null
 // for a single sketch.  capture: the launch sequence is being recorded into a
 // graph (no timing events, no synchronisation).
+// fixed: run in this scratch (null: the handle's own, allocated on demand); the
+// batch must then fit one chunk of it.
 int qs_insert_set(qs_sketch *s, const void *d_keys, const uint64_t *offs, const uint64_t *d_offs,
-                  hipStream_t stream, bool blocking, bool capture);
+                  hipStream_t stream, bool blocking, bool capture, const qs_scratch *fixed = nullptr);
 int qs_ensure_scratch(qs_sketch *s, uint64_t chunk);
+// allocation entries the insert scratch needs for a chunk of `chunk` keys
+uint32_t qs_trig_cap(const qs_sketch *s, uint64_t chunk);
+// Lay out a scratch for chunk keys and tcap allocations from base (256-byte
+// aligned pieces); returns the bytes it spans.  base may be null to size it.
+uint64_t qs_scratch_bytes(uint64_t chunk, uint32_t tcap, qs_scratch *sc, uint8_t *base);
 // segment kernel shapes the insert supports (group width x items per lane)
 bool qs_seg_shape_ok(uint32_t w, uint32_t ipl);
 // keys per chunk for a batch with host offsets offs

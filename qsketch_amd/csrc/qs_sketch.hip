@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -572,6 +573,67 @@ extern "C" int qs_insert(qs_sketch *s, const void *d_keys, size_t n)
 extern "C" int qs_insert_async(qs_sketch *s, const void *d_keys, size_t n, void *stream)
 {
     return insert_single(s, d_keys, n, (hipStream_t)stream, false);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free insert: the insert scratch (items, radix counts, digit
+// bytes, replay and long-segment lists, allocation records and their sort
+// buffers) is carved out of one static device array instead of the handle's
+// own buffers.  Each GPU holds its own copy of the array from module load to
+// process end; its address on a device comes from hipGetSymbolAddress, taken
+// once per device with that device current.  The array's name is never used
+// in host code beyond that call.
+// ---------------------------------------------------------------------------
+#define QS_NOALLOC_BYTES (33ull << 20)
+#define QS_NOALLOC_DEVICES 64
+
+static __device__ __align__(256) uint8_t scratch[QS_NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[QS_NOALLOC_DEVICES];
+
+// the array's device address on device dev (the current device)
+static int noalloc_scratch(int dev, uint8_t **out)
+{
+    if (dev < 0 || dev >= QS_NOALLOC_DEVICES)
+        return QS_ERR_PARAM;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+            return QS_ERR_HIP;
+        g_scratch_ptr[dev] = (uint8_t *)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return QS_OK;
+}
+
+extern "C" int qs_insert_noalloc(qs_sketch *s, const void *d_keys, size_t n, void *stream)
+{
+    if (!s || (!d_keys && n))
+        return QS_ERR_PARAM;
+    if (s->nsk != 1 || n > QS_NOALLOC_MAX_KEYS)
+        return QS_ERR_PARAM;
+    if (s->sticky_err)
+        return s->sticky_err;
+    if (n == 0)
+        return QS_OK;
+    const uint32_t tcap = qs_trig_cap(s, n);
+    if (tcap > QS_NOALLOC_MAX_ALLOCS)
+        return QS_ERR_PARAM;
+    int dev = -1;
+    CK(hipGetDevice(&dev));
+    if (dev != s->device)
+        return QS_ERR_PARAM; // the sketch lives on another device
+    uint8_t *base = nullptr;
+    int rc = noalloc_scratch(dev, &base);
+    if (rc)
+        return rc;
+    qs_scratch sc;
+    if (qs_scratch_bytes(n, tcap, &sc, base) > QS_NOALLOC_BYTES)
+        return QS_ERR_INTERNAL; // the limits above do not fit the array
+    const uint64_t offs[2] = {0, (uint64_t)n};
+    return qs_insert_set(s, d_keys, offs, nullptr, (hipStream_t)stream, false, false, &sc);
 }
 
 static int check_offs(const qs_sketch *s, const uint64_t *h_offs)
