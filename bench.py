@@ -203,8 +203,10 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_source": "FETCH_SIZE/WRITE_SIZE runs of this workload calibrated by a 1 GiB copy, "
-                              "profiles/insert_query_plain_zipf1.1_2e28_traffic.md" if traffic else None,
+            "traffic_source": "not measured in this run: bytes per key from the FETCH_SIZE/WRITE_SIZE "
+                              "counter runs of this workload (calibrated by a 1 GiB copy) in "
+                              "profiles/insert_query_plain_zipf1.1_2e28_traffic.md, times the keys "
+                              "of one launch" if traffic else None,
             "launch": "the launch sequence of one insert chunk (radix passes, k_segw, k_ordered, "
                       "k_assign, k_tail, ...; a 2^28-key insert is one chunk), timed with HIP events "
                       "around each kernel on its stream",
@@ -229,7 +231,8 @@ def main():
                   "long_segments_per_step": st["long_segments"],
                   "linked_buckets": st["linked_buckets"],
                   "pool_occupancy": round(st["linked_buckets"] / (a.n_high - 1), 4),
-                  "allocator_retries": 0},
+                  "allocator": "ids assigned in stream order by a sort of the triggering inserts; "
+                               "no compare-and-swap loop, so no retries"},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(keys, n_low, a.n_high, a.cpu_sample_log2)

@@ -28,6 +28,7 @@ class _Sk(C.Structure):
         ("key_bytes", C.c_uint32), ("semantics", C.c_uint32), ("seed", C.c_uint32),
         ("masks", C.POINTER(C.c_uint32)), ("low", C.POINTER(C.c_uint32)),
         ("high", C.POINTER(C.c_uint32)), ("next", C.c_uint32), ("n_triggers", C.c_uint64),
+        ("seed64", C.c_uint64),
     ]
 
 
@@ -201,7 +202,8 @@ class OracleSketch:
 
 class _Sl(C.Structure):
     _fields_ = [("n", C.c_uint64), ("m", C.c_uint32), ("n_masks", C.c_uint32), ("key_bytes", C.c_uint32),
-                ("seed", C.c_uint32), ("masks", C.POINTER(C.c_uint32)), ("table", C.POINTER(C.c_uint32))]
+                ("seed", C.c_uint32), ("masks", C.POINTER(C.c_uint32)), ("table", C.POINTER(C.c_uint32)),
+                ("seed64", C.c_uint64)]
 
 
 class OracleSingleLevel:

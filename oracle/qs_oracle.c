@@ -267,6 +267,7 @@ int orq_create_explicit(orq_sketch *s, uint64_t n_low, uint64_t n_high,
     s->key_bytes = key_bytes;
     s->semantics = semantics;
     s->seed = seed;
+    s->seed64 = seed;
     s->masks = (uint32_t *)malloc(sizeof(uint32_t) * n_masks);
     if (!s->masks)
         return ORQ_ERR_NOMEM;
@@ -286,10 +287,14 @@ int orq_create(orq_sketch *s, uint64_t n_low, uint64_t n_high, uint32_t n_masks,
     orq_minstd_seed_bytes(&mask_eng, buf, 128);
 
     /* Device_Seed::resize -> random(seed, seed_sz, 1, UINT_MAX): hash_mul_add
-     * sums key * seed[i], so the seeds act through their sum mod 2^32. */
-    uint32_t seed = 0;
+     * (device_hash_function.h:6-17) sums key * seed[i] in Hashed_T, so the
+     * seeds act through their sum: mod 2^32 for 32-bit keys (Hashed_T = uint,
+     * src/main.cu:5), mod 2^64 for 64-bit keys (the default Hashed_T =
+     * size_t, where key * seed[i] is a 64-bit product). */
+    uint64_t seed64 = 0;
     for (uint32_t i = 0; i < seed_sz; ++i)
-        seed += orq_uniform_u32(&seed_eng, 1u, 0xffffffffu);
+        seed64 += orq_uniform_u32(&seed_eng, 1u, 0xffffffffu);
+    const uint32_t seed = (uint32_t)seed64;
 
     uint8_t *mb = (uint8_t *)calloc((size_t)n_masks * 4, 1);
     uint32_t *masks = (uint32_t *)malloc(sizeof(uint32_t) * (n_masks ? n_masks : 1));
@@ -303,6 +308,7 @@ int orq_create(orq_sketch *s, uint64_t n_low, uint64_t n_high, uint32_t n_masks,
         masks[i] = (uint32_t)mb[4 * i] | ((uint32_t)mb[4 * i + 1] << 8) |
                    ((uint32_t)mb[4 * i + 2] << 16) | ((uint32_t)mb[4 * i + 3] << 24);
     int rc = orq_create_explicit(s, n_low, n_high, n_masks, seed, masks, key_bytes, semantics);
+    s->seed64 = seed64;
     free(mb);
     free(masks);
     return rc;
@@ -328,7 +334,7 @@ uint64_t orq_hash(const orq_sketch *s, uint64_t key)
 {
     if (s->key_bytes == 4)
         return (uint32_t)((uint32_t)key * s->seed);
-    return key * (uint64_t)s->seed;
+    return key * s->seed64;
 }
 
 static inline uint64_t key_at(const orq_sketch *s, const void *keys, size_t i)
@@ -634,9 +640,10 @@ int orq_sl_create(orq_sl *s, uint64_t n, uint32_t m, uint32_t n_masks, uint32_t 
     orq_minstd_seed_bytes(&seed_eng, buf, 128);
     orq_seed_buffer(rng_seed, 1, buf);
     orq_minstd_seed_bytes(&mask_eng, buf, 128);
-    uint32_t seed = 0;
+    uint64_t seed64 = 0; /* as in orq_create: 64-bit sum for 64-bit keys */
     for (uint32_t i = 0; i < seed_sz; ++i)
-        seed += orq_uniform_u32(&seed_eng, 1u, 0xffffffffu);
+        seed64 += orq_uniform_u32(&seed_eng, 1u, 0xffffffffu);
+    const uint32_t seed = (uint32_t)seed64;
     const uint32_t sz = (m + 7) / 8;
     uint8_t *mb = (uint8_t *)calloc((size_t)n_masks * sz, 1);
     s->masks = (uint32_t *)malloc(sizeof(uint32_t) * n_masks);
@@ -660,6 +667,7 @@ int orq_sl_create(orq_sl *s, uint64_t n, uint32_t m, uint32_t n_masks, uint32_t 
     s->n_masks = n_masks;
     s->key_bytes = key_bytes;
     s->seed = seed;
+    s->seed64 = seed64;
     return ORQ_OK;
 }
 
@@ -675,7 +683,7 @@ static inline uint64_t sl_hash(const orq_sl *s, const void *keys, size_t i)
 {
     if (s->key_bytes == 4)
         return (uint32_t)(((const uint32_t *)keys)[i] * s->seed);
-    return ((const uint64_t *)keys)[i] * (uint64_t)s->seed;
+    return ((const uint64_t *)keys)[i] * s->seed64;
 }
 
 void orq_sl_insert(orq_sl *s, const void *keys, size_t n)

@@ -64,6 +64,8 @@ typedef struct {
     uint32_t *high;       /* n_high * 32 words */
     uint32_t  next;       /* next high id, starts at 1025 */
     uint64_t  n_triggers; /* number of allocations so far */
+    uint64_t  seed64;     /* the seed sum in 64 bits: 64-bit keys hash as
+                             sum_i key * seed_i mod 2^64 = key * seed64 */
 } orq_sketch;
 
 /* Create with the reference's constructor call pattern (v3:544-563):
@@ -105,6 +107,7 @@ typedef struct {
     uint32_t  seed;
     uint32_t *masks;   /* bits 0..m-1 */
     uint32_t *table;   /* n * m counters */
+    uint64_t  seed64;  /* the seed sum in 64 bits (64-bit keys) */
 } orq_sl;
 int  orq_sl_create(orq_sl *s, uint64_t n, uint32_t m, uint32_t n_masks, uint32_t seed_sz,
                    uint32_t key_bytes, uint64_t rng_seed);

@@ -222,6 +222,42 @@ def _ptr(x) -> int:
     raise TypeError(f"unsupported buffer {type(x)!r}")
 
 
+def _keys_arg(keys, n, key_bytes: int, what: str):
+    """(device address, n) of a key buffer.  For a tensor, n (default: all of
+    it) must fit in it and its elements must be key_bytes wide; a raw address
+    needs an explicit n, and the caller vouches for its size."""
+    if hasattr(keys, "numel"):
+        n = keys.numel() if n is None else int(n)
+        if n < 0 or n > keys.numel():
+            raise ValueError(f"{what}: n = {n} but the key tensor holds {keys.numel()} elements")
+        if keys.element_size() != key_bytes:
+            raise ValueError(f"{what}: the sketch takes {key_bytes}-byte keys, the tensor has "
+                             f"{keys.element_size()}-byte elements ({keys.dtype})")
+        if keys.dtype.is_floating_point or keys.dtype.is_complex:
+            raise ValueError(f"{what}: keys must be an integer tensor, not {keys.dtype}")
+    elif n is None:
+        raise ValueError(f"{what}: n is required with a raw device address")
+    return _ptr(keys), int(n)
+
+
+def _counts_arg(counts, n: int, what: str, dtype_bytes: int = 4, floating: bool = False):
+    """Device address of an output buffer of at least n elements of a 32-bit
+    integer type (or float32 when floating)."""
+    if hasattr(counts, "numel"):
+        if counts.element_size() != dtype_bytes or counts.dtype.is_floating_point != floating:
+            want = "float32" if floating else "32-bit integer"
+            raise ValueError(f"{what}: the output must be a {want} tensor, not {counts.dtype}")
+        if counts.numel() < n:
+            raise ValueError(f"{what}: the output holds {counts.numel()} elements, {n} needed")
+    return _ptr(counts)
+
+
+def _offs_keys(o, keys, key_bytes: int, what: str):
+    """Check a batch's offsets against its key tensor; returns the key address."""
+    ptr, _ = _keys_arg(keys, int(o[-1]), key_bytes, what)
+    return ptr
+
+
 def default_params(expected_keys: int, wlf: float = 0.5) -> qs_params:
     """Reference sizing rule (device_sketch.h:371-376, lib/qsketch.cu:50-115)."""
     p = qs_params()
@@ -298,17 +334,17 @@ class Sketch_GPU_Sub_Warp_Level:
 
     # -- Sketch<Key_T, Count_T> interface (sketch.h:7-14) ------------------
     def insert(self, keys, keys_sz: int | None = None) -> int:
-        n = keys.numel() if keys_sz is None else int(keys_sz)
-        rc = _lib().qs_insert(self._h, _ptr(keys), n)
+        kp, n = _keys_arg(keys, keys_sz, self.key_bytes, "qs_insert")
+        rc = _lib().qs_insert(self._h, kp, n)
         check(rc, "qs_insert")
         return rc
 
     def search(self, keys, keys_sz: int | None = None, counts=None):
-        n = keys.numel() if keys_sz is None else int(keys_sz)
+        kp, n = _keys_arg(keys, keys_sz, self.key_bytes, "qs_query")
         if counts is None:
             import torch
             counts = torch.empty(n, dtype=torch.int32, device=keys.device)
-        check(_lib().qs_query(self._h, _ptr(keys), n, _ptr(counts)), "qs_query")
+        check(_lib().qs_query(self._h, kp, n, _counts_arg(counts, n, "qs_query")), "qs_query")
         return counts
 
     query = search
@@ -328,19 +364,21 @@ class Sketch_GPU_Sub_Warp_Level:
             pass
 
     # -- build additions -----------------------------------------------------
-    def insert_async(self, keys, n: int, stream: int = 0) -> None:
-        check(_lib().qs_insert_async(self._h, _ptr(keys), int(n), C.c_void_p(stream)), "qs_insert_async")
+    def insert_async(self, keys, n: int | None = None, stream: int = 0) -> None:
+        kp, n = _keys_arg(keys, n, self.key_bytes, "qs_insert_async")
+        check(_lib().qs_insert_async(self._h, kp, n, C.c_void_p(stream)), "qs_insert_async")
 
-    def search_async(self, keys, n: int, counts, stream: int = 0) -> None:
-        check(_lib().qs_query_async(self._h, _ptr(keys), int(n), _ptr(counts), C.c_void_p(stream)),
-              "qs_query_async")
+    def search_async(self, keys, n: int | None, counts, stream: int = 0) -> None:
+        kp, n = _keys_arg(keys, n, self.key_bytes, "qs_query_async")
+        check(_lib().qs_query_async(self._h, kp, n, _counts_arg(counts, n, "qs_query_async"),
+                                    C.c_void_p(stream)), "qs_query_async")
 
     def insert_noalloc(self, keys, n: int | None = None, stream: int = 0) -> None:
         """Stream-ordered insert whose scratch is the library's static device
         array (no allocation): n <= NOALLOC_MAX_KEYS, min(n_high, n) <=
         NOALLOC_MAX_ALLOCS."""
-        n = keys.numel() if n is None else int(n)
-        check(_lib().qs_insert_noalloc(self._h, _ptr(keys), n, C.c_void_p(stream)), "qs_insert_noalloc")
+        kp, n = _keys_arg(keys, n, self.key_bytes, "qs_insert_noalloc")
+        check(_lib().qs_insert_noalloc(self._h, kp, n, C.c_void_p(stream)), "qs_insert_noalloc")
 
     def sync(self) -> None:
         check(_lib().qs_sync(self._h), "qs_sync")
@@ -357,10 +395,10 @@ class Sketch_GPU_Sub_Warp_Level:
     def autotune(self, keys, n: int | None = None, reps: int = 2) -> dict:
         """Time all 16 segment-pass shapes on a twin sketch with these keys; keep the fastest."""
         import numpy as np
-        n = keys.numel() if n is None else int(n)
+        kp, n = _keys_arg(keys, n, self.key_bytes, "qs_autotune_insert")
         o = np.array([0, n], dtype=np.uint64)
         r = qs_tune_result()
-        rc = _lib().qs_autotune_insert(self._h, _ptr(keys), o.ctypes.data, None, int(reps), C.byref(r))
+        rc = _lib().qs_autotune_insert(self._h, kp, o.ctypes.data, None, int(reps), C.byref(r))
         table = [{"width": r.e[i].width, "items_per_lane": r.e[i].items_per_lane,
                   "seg_ms": round(r.e[i].seg_ms, 4), "insert_ms": round(r.e[i].insert_ms, 4)}
                  for i in range(r.n)]
@@ -370,10 +408,10 @@ class Sketch_GPU_Sub_Warp_Level:
     def capture_insert(self, keys, n: int | None = None) -> "Graph":
         """Record the insert of n keys at this address into a hipGraph."""
         import numpy as np
-        n = keys.numel() if n is None else int(n)
+        kp, n = _keys_arg(keys, n, self.key_bytes, "qs_graph_capture_insert")
         o = np.array([0, n], dtype=np.uint64)
         g = C.c_void_p()
-        check(_lib().qs_graph_capture_insert(self._h, _ptr(keys), o.ctypes.data, None, C.byref(g)),
+        check(_lib().qs_graph_capture_insert(self._h, kp, o.ctypes.data, None, C.byref(g)),
               "qs_graph_capture_insert")
         return Graph(g, self)
 
@@ -461,34 +499,53 @@ class SketchSet:
         check(_lib().qs_multi_create(arr, self.n, C.byref(h)), "qs_multi_create")
         self._h = h
 
-    def insert(self, keys, offs) -> None:
+    def _d_offs(self, d_offs, what):
+        if hasattr(d_offs, "numel") and (d_offs.numel() != self.n + 1 or d_offs.element_size() != 8):
+            raise ValueError(f"{what}: device offsets must be {self.n + 1} 64-bit values")
+        return _ptr(d_offs)
+
+    def _h_offs(self, offs, what):
         o = _offs_array(offs)
-        check(_lib().qs_multi_insert(self._h, _ptr(keys), o.ctypes.data), "qs_multi_insert")
+        if o.size != self.n + 1:
+            raise ValueError(f"{what}: {self.n + 1} offsets needed, {o.size} given")
+        return o
+
+    def insert(self, keys, offs) -> None:
+        o = self._h_offs(offs, "qs_multi_insert")
+        kp = _offs_keys(o, keys, self.key_bytes, "qs_multi_insert")
+        check(_lib().qs_multi_insert(self._h, kp, o.ctypes.data), "qs_multi_insert")
 
     def insert_async(self, keys, offs, d_offs, stream: int = 0) -> None:
-        o = _offs_array(offs)
-        check(_lib().qs_multi_insert_async(self._h, _ptr(keys), o.ctypes.data, _ptr(d_offs),
+        o = self._h_offs(offs, "qs_multi_insert_async")
+        kp = _offs_keys(o, keys, self.key_bytes, "qs_multi_insert_async")
+        check(_lib().qs_multi_insert_async(self._h, kp, o.ctypes.data, self._d_offs(d_offs, "qs_multi_insert_async"),
                                            C.c_void_p(stream)), "qs_multi_insert_async")
 
     def search(self, keys, offs, counts=None):
         import torch
-        o = _offs_array(offs)
+        o = self._h_offs(offs, "qs_multi_query")
+        kp = _offs_keys(o, keys, self.key_bytes, "qs_multi_query")
         if counts is None:
             counts = torch.empty(int(o[-1]), dtype=torch.int32, device=keys.device)
-        check(_lib().qs_multi_query(self._h, _ptr(keys), o.ctypes.data, _ptr(counts)), "qs_multi_query")
+        check(_lib().qs_multi_query(self._h, kp, o.ctypes.data, _counts_arg(counts, int(o[-1]), "qs_multi_query")),
+              "qs_multi_query")
         return counts
 
     query = search
 
     def search_async(self, keys, d_offs, n: int, counts, stream: int = 0) -> None:
-        check(_lib().qs_multi_query_async(self._h, _ptr(keys), _ptr(d_offs), int(n), _ptr(counts),
-                                          C.c_void_p(stream)), "qs_multi_query_async")
+        kp, n = _keys_arg(keys, n, self.key_bytes, "qs_multi_query_async")
+        check(_lib().qs_multi_query_async(self._h, kp, self._d_offs(d_offs, "qs_multi_query_async"), n,
+                                          _counts_arg(counts, n, "qs_multi_query_async"), C.c_void_p(stream)),
+              "qs_multi_query_async")
 
     def capture_insert(self, keys, offs, d_offs) -> "Graph":
         """Record the insert of one batch layout into a hipGraph (see Graph)."""
-        o = _offs_array(offs)
+        o = self._h_offs(offs, "qs_graph_capture_insert")
+        kp = _offs_keys(o, keys, self.key_bytes, "qs_graph_capture_insert")
         g = C.c_void_p()
-        check(_lib().qs_graph_capture_insert(self._h, _ptr(keys), o.ctypes.data, _ptr(d_offs), C.byref(g)),
+        check(_lib().qs_graph_capture_insert(self._h, kp, o.ctypes.data, self._d_offs(d_offs, "qs_graph_capture_insert"),
+                                             C.byref(g)),
               "qs_graph_capture_insert")
         return Graph(g, self)
 
@@ -509,9 +566,11 @@ class SketchSet:
 
     def autotune(self, keys, offs, d_offs, reps: int = 2) -> dict:
         """Time all 16 shapes on a twin set with this batch; keep the fastest."""
-        o = _offs_array(offs)
+        o = self._h_offs(offs, "qs_autotune_insert")
+        kp = _offs_keys(o, keys, self.key_bytes, "qs_autotune_insert")
         r = qs_tune_result()
-        rc = _lib().qs_autotune_insert(self._h, _ptr(keys), o.ctypes.data, _ptr(d_offs), int(reps), C.byref(r))
+        rc = _lib().qs_autotune_insert(self._h, kp, o.ctypes.data, self._d_offs(d_offs, "qs_autotune_insert"),
+                                       int(reps), C.byref(r))
         table = [{"width": r.e[i].width, "items_per_lane": r.e[i].items_per_lane,
                   "seg_ms": round(r.e[i].seg_ms, 4), "insert_ms": round(r.e[i].insert_ms, 4),
                   "checksum": r.e[i].checksum} for i in range(r.n)]
@@ -580,16 +639,16 @@ class Sketch_GPU_Sub_Warp:
         self.n, self.m, self.n_masks, self.key_bytes = int(n), int(m), int(n_hash_mask_table), int(key_bytes)
 
     def insert(self, keys, keys_sz: int | None = None) -> int:
-        n = keys.numel() if keys_sz is None else int(keys_sz)
-        check(_lib().qs_sl_insert(self._h, _ptr(keys), n), "qs_sl_insert")
+        kp, n = _keys_arg(keys, keys_sz, self.key_bytes, "qs_sl_insert")
+        check(_lib().qs_sl_insert(self._h, kp, n), "qs_sl_insert")
         return 0
 
     def search(self, keys, keys_sz: int | None = None, counts=None):
         import torch
-        n = keys.numel() if keys_sz is None else int(keys_sz)
+        kp, n = _keys_arg(keys, keys_sz, self.key_bytes, "qs_sl_query")
         if counts is None:
             counts = torch.empty(n, dtype=torch.int32, device=keys.device)
-        check(_lib().qs_sl_query(self._h, _ptr(keys), n, _ptr(counts)), "qs_sl_query")
+        check(_lib().qs_sl_query(self._h, kp, n, _counts_arg(counts, n, "qs_sl_query")), "qs_sl_query")
         return counts
 
     query = search
@@ -647,28 +706,36 @@ class LearnedSketch:
         assert a.size == MLP_PARAMS
         check(_lib().qs_mlp_set_params(self._h, a.ctypes.data), "qs_mlp_set_params")
 
+    def _pairs(self, keys, targets, what):
+        kp, n = _keys_arg(keys, None, self.key_bytes, what)
+        if n < 1 or n > 0xFFFFFFFF:
+            raise ValueError(f"{what}: 1 .. 2^32 - 1 pairs per batch, {n} given")
+        return kp, _counts_arg(targets, n, what, floating=True), n
+
     def grad(self, keys, targets):
         """(loss, FP32 gradient) of a batch; no update."""
         import numpy as np
+        kp, tp, n = self._pairs(keys, targets, "qs_mlp_grad")
         g = np.zeros(MLP_PARAMS, np.float32)
         loss = C.c_float()
-        check(_lib().qs_mlp_grad(self._h, _ptr(keys), _ptr(targets), keys.numel(), C.byref(loss), g.ctypes.data),
-              "qs_mlp_grad")
+        check(_lib().qs_mlp_grad(self._h, kp, tp, n, C.byref(loss), g.ctypes.data), "qs_mlp_grad")
         return loss.value, g
 
     def step(self, keys, targets, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999,
              eps: float = 1e-8, want_loss: bool = True):
+        kp, tp, n = self._pairs(keys, targets, "qs_mlp_step")
         loss = C.c_float()
-        check(_lib().qs_mlp_step(self._h, _ptr(keys), _ptr(targets), keys.numel(), lr, beta1, beta2, eps,
+        check(_lib().qs_mlp_step(self._h, kp, tp, n, lr, beta1, beta2, eps,
                                  C.byref(loss) if want_loss else None, None), "qs_mlp_step")
         return loss.value if want_loss else None
 
     def score(self, keys, mfma: bool = True, out=None):
         import torch
+        kp, n = _keys_arg(keys, None, self.key_bytes, "qs_mlp_score")
         if out is None:
-            out = torch.empty(keys.numel(), dtype=torch.float32, device=keys.device)
-        check(_lib().qs_mlp_score(self._h, _ptr(keys), keys.numel(), _ptr(out), 1 if mfma else 0),
-              "qs_mlp_score")
+            out = torch.empty(n, dtype=torch.float32, device=keys.device)
+        check(_lib().qs_mlp_score(self._h, kp, n, _counts_arg(out, n, "qs_mlp_score", floating=True),
+                                  1 if mfma else 0), "qs_mlp_score")
         return out
 
     # -- routing ---------------------------------------------------------------
@@ -682,17 +749,18 @@ class LearnedSketch:
 
     def insert(self, keys, n: int | None = None) -> int:
         """Insert; returns how many keys went to the exact table."""
-        n = keys.numel() if n is None else int(n)
+        kp, n = _keys_arg(keys, n, self.key_bytes, "qs_learned_insert")
         nh = C.c_uint64()
-        check(_lib().qs_learned_insert(self._h, _ptr(keys), n, C.byref(nh)), "qs_learned_insert")
+        check(_lib().qs_learned_insert(self._h, kp, n, C.byref(nh)), "qs_learned_insert")
         return nh.value
 
     def search(self, keys, n: int | None = None, counts=None):
         import torch
-        n = keys.numel() if n is None else int(n)
+        kp, n = _keys_arg(keys, n, self.key_bytes, "qs_learned_query")
         if counts is None:
             counts = torch.empty(n, dtype=torch.int32, device=keys.device)
-        check(_lib().qs_learned_query(self._h, _ptr(keys), n, _ptr(counts)), "qs_learned_query")
+        check(_lib().qs_learned_query(self._h, kp, n, _counts_arg(counts, n, "qs_learned_query")),
+              "qs_learned_query")
         return counts
 
     query = search
@@ -776,16 +844,23 @@ class Zipf:
         self._h = h
         self.alpha, self.universe = alpha, universe
 
-    def __del__(self):
+    def destroy(self) -> None:
         if getattr(self, "_h", None):
             _lib().qs_zipf_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
     def keys_u32(self, seed: int, n: int, out=None, offset: int = 0, device: str = "cuda"):
         import torch
         if out is None:
             out = torch.empty(n, dtype=torch.int32, device=device)
-        check(_lib().qs_gen_zipf_u32(self._h, seed, offset, n, _ptr(out), None), "qs_gen_zipf_u32")
+        check(_lib().qs_gen_zipf_u32(self._h, seed, offset, n, _counts_arg(out, n, "qs_gen_zipf_u32"), None),
+              "qs_gen_zipf_u32")
         return out
 
     def keys_u32_host(self, seed: int, n: int, offset: int = 0):
@@ -805,7 +880,8 @@ class Zipf:
         import torch
         if out is None:
             out = torch.empty(n, dtype=torch.int64, device=device)
-        check(_lib().qs_gen_zipf_u64_noisy(self._h, seed, offset, n, float(sigma), _ptr(out), None),
+        check(_lib().qs_gen_zipf_u64_noisy(self._h, seed, offset, n, float(sigma),
+                                           _counts_arg(out, n, "qs_gen_zipf_u64_noisy", dtype_bytes=8), None),
               "qs_gen_zipf_u64_noisy")
         return out
 
@@ -821,7 +897,8 @@ def uniform_u32(seed: int, n: int, out=None, offset: int = 0, device: str = "cud
     import torch
     if out is None:
         out = torch.empty(n, dtype=torch.int32, device=device)
-    check(_lib().qs_gen_uniform_u32(seed, offset, n, _ptr(out), None), "qs_gen_uniform_u32")
+    check(_lib().qs_gen_uniform_u32(seed, offset, n, _counts_arg(out, n, "qs_gen_uniform_u32"), None),
+          "qs_gen_uniform_u32")
     return out
 
 

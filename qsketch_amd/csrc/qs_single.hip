@@ -182,6 +182,8 @@ extern "C" int qs_sl_create(uint64_t n, uint32_t m, uint32_t seed_sz, uint32_t n
         return QS_ERR_PARAM;
     if (key_bytes == 4 && n > 0xffffffffull)
         return QS_ERR_PARAM;
+    if (key_bytes == 8 && seed_sz > 1)
+        return QS_ERR_PARAM; // 64-bit keys sum the seeds in 64 bits (device_hash_function.h:6-17)
     if (n * m > (1ull << 36))
         return QS_ERR_PARAM;
     int ndev = 0;

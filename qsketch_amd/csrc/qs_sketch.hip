@@ -305,6 +305,11 @@ static int validate_params(const qs_params *p)
         return QS_ERR_PARAM;
     if (p->seed_sz < 1 || (p->semantics != QS_LITERAL && p->semantics != QS_PAPER))
         return QS_ERR_PARAM;
+    // hash_mul_add sums key * seed_i in 64 bits for 64-bit keys
+    // (device_hash_function.h:6-17): the seed sum can exceed 32 bits, and the
+    // device hash multiplies by a 32-bit seed, so 64-bit keys take one seed
+    if (p->key_bytes == 8 && p->seed_sz > 1 && p->hash_seed == 0)
+        return QS_ERR_PARAM;
     return QS_OK;
 }
 

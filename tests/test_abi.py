@@ -151,3 +151,75 @@ def test_largest_prime_at_most():
     # agrees with the reference's prime table below its largest entry (3 * 2^27)
     for k in range(2, 29):
         assert q.largest_prime_at_most(1 << k) == po.find_greatest_prime(1 << k)
+
+
+# ---------------------------------------------------------------------------
+# argument checks of the Python wrappers (no GPU: they fail before any call)
+# ---------------------------------------------------------------------------
+def test_wrapper_rejects_buffers_that_do_not_fit():
+    import torch
+    import qsketch_amd as q
+    k32 = torch.zeros(10, dtype=torch.int32)
+    k64 = torch.zeros(10, dtype=torch.int64)
+    with pytest.raises(ValueError, match="holds 10"):
+        q._keys_arg(k32, 11, 4, "x")            # n beyond the tensor
+    with pytest.raises(ValueError, match="4-byte keys"):
+        q._keys_arg(k64, None, 4, "x")          # u64 tensor on a u32 sketch
+    with pytest.raises(ValueError, match="8-byte keys"):
+        q._keys_arg(k32, None, 8, "x")          # u32 tensor on a u64 sketch
+    with pytest.raises(ValueError, match="integer"):
+        q._keys_arg(torch.zeros(10, dtype=torch.float32), None, 4, "x")
+    with pytest.raises(ValueError, match="raw device address"):
+        q._keys_arg(12345, None, 4, "x")
+    with pytest.raises(ValueError, match="32-bit integer"):
+        q._counts_arg(torch.zeros(10, dtype=torch.int64), 10, "x")
+    with pytest.raises(ValueError, match="32-bit integer"):
+        q._counts_arg(torch.zeros(10, dtype=torch.float32), 10, "x")
+    with pytest.raises(ValueError, match="9 elements, 10 needed"):
+        q._counts_arg(torch.zeros(9, dtype=torch.int32), 10, "x")
+    with pytest.raises(ValueError, match="float32"):
+        q._counts_arg(torch.zeros(10, dtype=torch.int32), 10, "x", floating=True)
+    o = q._offs_array([0, 4, 12])
+    with pytest.raises(ValueError, match="n = 12"):
+        q._offs_keys(o, k32, 4, "x")            # offs[-1] beyond the keys
+    with pytest.raises(ValueError, match="non-decreasing"):
+        q._offs_array([0, 5, 3])
+    # the checks pass for fitting buffers; the device check comes after them
+    with pytest.raises(ValueError, match="device tensors"):
+        q._keys_arg(k32, 10, 4, "x")
+
+
+def test_u64_keys_take_one_seed():
+    """hash_mul_add sums key * seed_i in 64 bits for 64-bit keys
+    (device_hash_function.h:6-17); the device hash multiplies by one 32-bit
+    seed, so the library refuses seed_sz > 1 with 64-bit keys."""
+    import qsketch_amd as q
+    L = q.lib()
+    p = q.make_params(1021, 1024, key_bytes=8, seed_sz=2)
+    h = C.c_void_p()
+    assert L.qs_create(C.byref(p), C.byref(h)) == q.QS_ERR_PARAM
+    assert L.qs_sl_create(1021, 8, 2, 1021, 8, 1, C.byref(h)) == q.QS_ERR_PARAM
+
+
+def test_oracle_u64_hash_is_hash_mul_add_in_64_bits():
+    """The oracle's 64-bit hash with two seeds equals sum_i key * seed_i mod
+    2^64, the functor's arithmetic restated with Python integers; a 32-bit sum
+    of the seeds (the plausible mistake) gives other buckets."""
+    from oracle import pyoracle as po
+    eng = po.Minstd(seed_bytes=po.seed_buffer(2, 0))
+    seeds = [eng.uniform(1, 0xFFFFFFFF) for _ in range(2)]
+    assert sum(seeds) >= 1 << 32  # rng_seed 2: the sum needs 33 bits
+    o = po.OracleSketch(1000003, 1024, 1021, seed_sz=2, key_bytes=8, rng_seed=2)
+    assert o.s.seed64 == sum(seeds)
+    rs = np.random.default_rng(4)
+    keys = rs.integers(0, 1 << 63, 2000, dtype=np.int64).view(np.uint64) * np.uint64(2) + np.uint64(1)
+    o.insert(keys)
+    want = np.zeros(o.s.n_low * 8, np.uint32)
+    for k in keys.tolist():
+        h = sum(k * s for s in seeds) % (1 << 64)
+        b, mu = h % o.s.n_low, h % 1021
+        m = int(o.masks()[mu])
+        for j in range(28):
+            if m >> j & 1:
+                want[b * 8 + j // 4] += 1 << (8 * (j % 4))
+    assert np.array_equal(o.low(), want)

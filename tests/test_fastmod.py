@@ -28,3 +28,15 @@ def test_item_packing_fields_and_order_keys():
     r = subprocess.run([PACK], stdout=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 0, r.stdout
     assert "pack violations: 0" in r.stdout
+
+
+def test_fastmod_exhaustive_over_all_dividends():
+    """Every 32-bit dividend for the divisors the workloads use (1021 masks and
+    the reference sizing rule's tables: 3,145,721, 100,663,291, 201,326,557,
+    402,653,171 buckets); about 45 s on one core."""
+    if not os.path.exists(EXE):
+        from qsketch_amd import build as b
+        b.build_native_tests()
+    r = subprocess.run([EXE, "full"], stdout=subprocess.PIPE, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout
+    assert "mismatches: 0" in r.stdout
