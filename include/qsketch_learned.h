@@ -12,8 +12,24 @@
  * Training is the caller's loop over minibatches: qs_mlp_step computes the
  * mean-squared-error gradients of a batch (hand-written forward and backward
  * kernels) and applies one Adam step to FP32 master weights, refreshing the
- * BF16 copies the kernels read.  Targets are what the plain sketch estimated
+ * BF16 copies the kernels read.  Targets are what a plain sketch estimated
  * for keys of an earlier window (e.g. log2(1 + estimate)).
+ *
+ * Which part counts a key is a function of the network and the threshold, so
+ * both are fixed while the form holds data: qs_mlp_set_params, qs_mlp_step
+ * and qs_learned_set_threshold return QS_ERR_STATE after an insert until
+ * qs_learned_clear.  (Changing them with data in would let a query look for a
+ * key in the other part than its inserts went to.)  Train a fresh or cleared
+ * form, or a separate one, and copy the weights with get/set_params.
+ *
+ * A full table does not lose keys: a key the table has no slot for (the
+ * table is full, or its probe window is) is counted in the sketch instead,
+ * with all its occurrences of that insert call and later ones, in stream
+ * order.  Every key is thus counted entirely in one part.  A query returns
+ * the table count of a key found there, the sketch estimate of a light key
+ * or of a heavy key missing from the table once the table has filled, and 0
+ * for a heavy key missing from a table that never filled (no insert of it
+ * has happened).
  *
  * Parameter layout (FP32, 8385 values): W1[64][64] (row = hidden unit,
  * column = key bit), b1[64], W2[64][64] (row = layer-2 unit, column = layer-1
@@ -33,7 +49,7 @@ extern "C" {
 #endif
 
 #define QS_MLP_PARAMS 8385
-#define QS_ERR_TABLE  -7  /* exact table full: more routed keys than it holds */
+#define QS_ERR_TABLE  -7  /* reserved (an exact table that fills no longer fails an insert) */
 
 typedef struct qs_learned qs_learned;
 
@@ -80,6 +96,9 @@ int qs_learned_clear(qs_learned *L);
 /* Table contents: up to cap (key, count) pairs into host arrays; returns the
  * number of occupied slots in *n_used. */
 int qs_learned_table(qs_learned *L, uint64_t *h_keys, uint32_t *h_counts, uint64_t cap, uint64_t *n_used);
+/* 1 once the table has turned a key away to the sketch (until clear), else 0;
+ * negative on a bad handle. */
+int qs_learned_table_full(const qs_learned *L);
 /* Bytes of the table, the MLP (BF16 weights + FP32 biases) and the sketch. */
 int qs_learned_memory(const qs_learned *L, uint64_t *table_bytes, uint64_t *mlp_bytes, uint64_t *sketch_bytes);
 

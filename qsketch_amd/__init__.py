@@ -142,6 +142,7 @@ _SIGS = {
     "qs_learned_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "qs_learned_clear": (C.c_int, [C.c_void_p]),
     "qs_learned_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "qs_learned_table_full": (C.c_int, [C.c_void_p]),
     "qs_learned_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64)]),
     # single-level sub-warp sketch (include/qsketch_single.h)
@@ -777,6 +778,14 @@ class LearnedSketch:
         check(_lib().qs_learned_table(self._h, k.ctypes.data, c.ctypes.data, cap, C.byref(used)),
               "qs_learned_table")
         return k[:used.value], c[:used.value]
+
+    @property
+    def table_full(self) -> bool:
+        """True once the table has turned a key away to the sketch (until clear)."""
+        r = _lib().qs_learned_table_full(self._h)
+        if r < 0:
+            check(r, "qs_learned_table_full")
+        return bool(r)
 
     def memory(self) -> dict:
         t, m, s = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -9,7 +9,9 @@ metrics (include/analyze/analyze.h:7-64):
   * the number of under-estimates -- the reference's Count_Greater tripwire
     (analyze.h:49-64 counts est - true that wrapped below zero);
   * ARE = mean |est - true| / true and AAE = mean |est - true| over the queried
-    distinct keys (the metrics of the learned-oracle comparison).
+    distinct keys (the metrics of the learned-oracle comparison).  Queried keys
+    that were never inserted (true = 0) count in AAE and the under-estimate
+    tally but not in ARE or the mean ratio, where they would divide by zero.
 """
 from __future__ import annotations
 
@@ -26,16 +28,18 @@ def metrics(est, true) -> dict:
     """est, true: tensors of the same length (any integer dtype)."""
     import torch
     e = est.to(torch.int64) & 0xFFFFFFFF
-    t = true.to(torch.int64)
+    t = true.to(torch.int64).to(e.device)
     d = (e - t).abs().to(torch.float64)
-    tf = t.to(torch.float64)
+    pos = t > 0
+    tf = t[pos].to(torch.float64)
     return {
         "keys": int(t.numel()),
-        "are": float((d / tf).mean()),
-        "aae": float(d.mean()),
-        "mean_est_over_true_pct": float((e.to(torch.float64) / tf).mean() * 100.0),
+        "are": float((d[pos] / tf).mean()) if tf.numel() else 0.0,
+        "aae": float(d.mean()) if d.numel() else 0.0,
+        "mean_est_over_true_pct": float((e[pos].to(torch.float64) / tf).mean() * 100.0) if tf.numel() else 0.0,
         "max_abs_error": int(d.max()) if d.numel() else 0,
         "under_estimates": int((e < t).sum()),
+        "absent_keys": int((~pos).sum()),
     }
 
 

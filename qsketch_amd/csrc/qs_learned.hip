@@ -18,6 +18,10 @@
 //               in a per-block LDS table and flushed to the exact GPU table with
 //               one atomic per distinct key and block; the others are flagged
 //               for a stable compaction into the sketch's input
+//   k_reflag    after a routing pass the table could not hold: heavy keys
+//               without a slot go to the sketch instead
+//   k_learned_query  score + answer in one pass: table lookup for heavy
+//               keys, the sketch's bucket read only for the others
 #include "qs_internal.h"
 
 #include "../../include/qsketch_learned.h"
@@ -64,7 +68,9 @@ struct qs_learned {
     uint64_t tcap;
     uint64_t *d_tkeys;
     uint32_t *d_tcnt;
-    uint32_t *d_err;
+    uint32_t *d_err;   // set by k_route when the table turned a key away
+    uint64_t inserted; // keys inserted since create / clear
+    uint32_t full;     // the table has turned a key away since create / clear
     // routing scratch
     uint64_t rcap;     // keys
     uint8_t *d_flag;   // 1 = goes to the sketch
@@ -651,33 +657,125 @@ __global__ __launch_bounds__(ROUTE_THREADS) void k_light_scatter(const KeyT<KB> 
     }
 }
 
-// query: the table's exact count for keys routed to it (0 if absent: such a
-// key was never inserted), the sketch estimate (already in out) otherwise
+// After a routing pass in which the table turned keys away (*err set): every
+// key flagged heavy that is not in the table goes to the sketch instead, and
+// the tile's light count is redone.  A key is either in the table for all of
+// its occurrences or for none: slots only fill, so a key's probe window that
+// had no room for it never gains room, and once the key holds a slot every
+// later probe finds it.  Blocks exit at once when *err is clear.
 template <int KB>
-__global__ __launch_bounds__(256) void k_route_query(const KeyT<KB> *__restrict__ keys, uint64_t n,
-                                                     const uint16_t *__restrict__ pb, const float *__restrict__ p,
-                                                     float thr, const uint64_t *__restrict__ tkeys,
-                                                     const uint32_t *__restrict__ tcnt, uint32_t tlog2,
-                                                     uint32_t *__restrict__ out)
+__global__ __launch_bounds__(ROUTE_THREADS) void k_reflag(const KeyT<KB> *__restrict__ keys, uint64_t n,
+                                                          const uint64_t *__restrict__ tkeys, uint32_t tlog2,
+                                                          uint8_t *__restrict__ flag, uint32_t *__restrict__ tiles,
+                                                          const uint32_t *__restrict__ err)
+{
+    if (*err == 0)
+        return;
+    __shared__ uint32_t s_light;
+    if (threadIdx.x == 0)
+        s_light = 0;
+    __syncthreads();
+    const uint64_t tb = (uint64_t)blockIdx.x * ROUTE_TILE;
+    uint32_t light = 0;
+    for (uint32_t st = 0; st < ROUTE_TILE / ROUTE_THREADS; ++st) {
+        const uint64_t i = tb + (uint64_t)st * ROUTE_THREADS + threadIdx.x;
+        if (i >= n)
+            break;
+        uint8_t f = flag[i];
+        if (!f && t_find(tkeys, tlog2, key64<KB>(keys, i)) == T_NONE) {
+            f = 1;
+            flag[i] = 1;
+        }
+        light += f;
+    }
+    atomicAdd(&s_light, light);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        tiles[blockIdx.x] = s_light;
+}
+
+// Query of the learned form in one pass: a wavefront scores 32 keys on the
+// matrix cores (lanes 0..31 hold the scores), then each key is answered from
+// the part that counted it.  Heavy keys look up the exact table; the others
+// read their sketch bucket as k_query does, one 16-byte half per lane, with
+// lane l and lane l + 32 sharing key l.  No sketch read is spent on a key the
+// table answers.
+#define LQ_SKETCH 0xffffffffu // answer from the sketch
+#define LQ_ZERO 0xfffffffeu   // heavy, not in a table that never filled: never inserted
+
+__device__ static inline uint32_t lq_sel4(const uint4 &v, uint32_t k)
+{
+    const uint32_t lo = (k & 1u) ? v.y : v.x;
+    const uint32_t hi = (k & 1u) ? v.w : v.z;
+    return (k & 2u) ? hi : lo;
+}
+
+typedef unsigned int lq_u32x4 __attribute__((ext_vector_type(4)));
+
+template <int KB>
+__global__ __launch_bounds__(256) void k_learned_query(const KeyT<KB> *__restrict__ keys, uint64_t n,
+                                                       const uint16_t *__restrict__ pb, const float *__restrict__ p,
+                                                       float thr, const uint64_t *__restrict__ tkeys,
+                                                       const uint32_t *__restrict__ tcnt, uint32_t tlog2,
+                                                       uint32_t table_full, qs_sdesc d0,
+                                                       const uint32_t *__restrict__ low,
+                                                       const uint32_t *__restrict__ high,
+                                                       const uint32_t *__restrict__ masks,
+                                                       uint32_t *__restrict__ out)
 {
     __shared__ mlp_lds S;
-    load_mlp(S, pb, p);
-    const uint32_t lane = threadIdx.x & 63u;
+    __shared__ uint32_t s_masks[QS_MAX_MASKS];
+    for (uint32_t e = threadIdx.x; e < d0.n_masks; e += blockDim.x)
+        s_masks[e] = masks[e];
+    load_mlp(S, pb, p); // ends with a barrier
+    const uint32_t lane = threadIdx.x & 63u, l32 = lane & 31u, half = lane >> 5;
     const uint64_t gw = ((uint64_t)blockIdx.x * 256u + threadIdx.x) >> 6;
     const uint64_t nw = ((uint64_t)gridDim.x * 256u) >> 6;
     if (n == 0)
         return;
-    uint64_t xn = key_at<KB>(keys, gw * 32u + (lane & 31u), n);
+    uint64_t xn = key_at<KB>(keys, gw * 32u + l32, n);
     for (uint64_t base = gw * 32u; base < n; base += nw * 32u) {
-        const uint64_t i = base + (lane & 31u);
+        const uint64_t i = base + l32;
         const bool valid = i < n;
         const uint64_t x = xn;
         xn = key_at<KB>(keys, i + nw * 32u, n); // the next tile's key, loaded while this one is scored
         const float y = mlp_mfma(S, x);
+        uint32_t code = LQ_SKETCH;
         if (lane < 32u && valid && y >= thr && x != T_EMPTY) {
             const uint32_t s = t_find(tkeys, tlog2, x);
-            out[i] = s == T_NONE ? 0u : tcnt[s];
+            code = s != T_NONE ? s : (table_full ? LQ_SKETCH : LQ_ZERO);
         }
+        code = (uint32_t)__shfl((int)code, (int)l32, 64); // lane l + 32 takes key l's answer
+        const bool need = valid && code == LQ_SKETCH;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t Mh = 0;
+        if (need) {
+            const KeyT<KB> key = (KeyT<KB>)x;
+            const uint32_t M = s_masks[qs_maskidx_d(key, d0)];
+            const uint32_t b = qs_bucket_d(key, d0);
+            const lq_u32x4 vv =
+                __builtin_nontemporal_load(reinterpret_cast<const lq_u32x4 *>(low + (size_t)b * 8u) + half);
+            v = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+            Mh = half ? (M >> 16) : (M & 0xffffu);
+        }
+        uint32_t lo = 0xffffffffu;
+        for (uint32_t m = Mh; m; m &= m - 1u) {
+            const uint32_t jj = (uint32_t)__builtin_ctz(m);
+            lo = min(lo, (lq_sel4(v, jj >> 2) >> ((jj & 3u) * 8u)) & 0xffu);
+        }
+        // both halves of a key take the same path, so the exchanges stay in step
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, 32, 64));
+        const uint32_t wo = (uint32_t)__shfl_xor((int)v.w, 32, 64);
+        const uint32_t w = half ? v.w : wo;
+        uint32_t hi = 0xffffffffu;
+        if (need && w > QS_LINK_START) {
+            const uint32_t *g = high + (size_t)(w - QS_LINK_START) * 32u + 16u * half;
+            for (uint32_t m = Mh; m; m &= m - 1u)
+                hi = min(hi, g[__builtin_ctz(m)]);
+        }
+        hi = min(hi, (uint32_t)__shfl_xor((int)hi, 32, 64));
+        if (lane < 32u && valid)
+            out[i] = code == LQ_SKETCH ? lo + (w > QS_LINK_START ? hi : 0u) : code == LQ_ZERO ? 0u : tcnt[code];
     }
 }
 
@@ -792,6 +890,8 @@ extern "C" int qs_mlp_set_params(qs_learned *L, const float *hp)
 {
     if (!L || !hp)
         return QS_ERR_PARAM;
+    if (L->inserted)
+        return QS_ERR_STATE; // the routing of the keys already in would change
     CK(hipDeviceSynchronize());
     CK(hipMemcpy(L->d_p, hp, NPAR * 4, hipMemcpyHostToDevice));
     CK(hipMemset(L->d_m, 0, NPAR * 4));
@@ -843,6 +943,8 @@ extern "C" int qs_mlp_step(qs_learned *L, const void *d_keys, const float *d_t, 
 {
     if (!L || !d_keys || !d_t || n == 0)
         return QS_ERR_PARAM;
+    if (L->inserted)
+        return QS_ERR_STATE; // the routing of the keys already in would change
     int rc = grad_launch(L, d_keys, d_t, n);
     if (rc)
         return rc;
@@ -893,6 +995,8 @@ extern "C" int qs_learned_set_threshold(qs_learned *L, float thr)
 {
     if (!L || std::isnan(thr))
         return QS_ERR_PARAM;
+    if (L->inserted && thr != L->thr)
+        return QS_ERR_STATE; // the routing of the keys already in would change
     L->thr = thr;
     return QS_OK;
 }
@@ -914,6 +1018,8 @@ extern "C" int qs_learned_clear(qs_learned *L)
     CK(hipGetLastError());
     CK(hipMemset(L->d_err, 0, 4));
     CK(hipDeviceSynchronize());
+    L->inserted = 0;
+    L->full = 0;
     return QS_OK;
 }
 
@@ -943,8 +1049,12 @@ static int learned_insert(qs_learned *L, const KeyT<KB> *keys, uint64_t n, uint6
     if (rc)
         return rc;
     const uint32_t tiles = (uint32_t)((n + ROUTE_TILE - 1) / ROUTE_TILE);
+    CK(hipMemsetAsync(L->d_err, 0, 4, 0)); // this call's flag
     hipLaunchKernelGGL(k_route<KB>, dim3(tiles), dim3(ROUTE_THREADS), 0, 0, keys, n, L->d_pb, L->d_p, L->thr,
                        L->d_tkeys, L->d_tcnt, L->tlog2, L->d_flag, L->d_tiles, L->d_err);
+    // keys the table turned away go to the sketch (no work unless it did)
+    hipLaunchKernelGGL(k_reflag<KB>, dim3(tiles), dim3(ROUTE_THREADS), 0, 0, keys, n, L->d_tkeys, L->tlog2,
+                       L->d_flag, L->d_tiles, L->d_err);
     qs_scan_digits(L->d_tiles, tiles, 1, L->d_tiles + tiles, 0);
     hipLaunchKernelGGL(k_light_scatter<KB>, dim3(tiles), dim3(ROUTE_THREADS), 0, 0, keys, n, L->d_flag, L->d_tiles,
                        (KeyT<KB> *)L->d_light);
@@ -953,7 +1063,8 @@ static int learned_insert(qs_learned *L, const KeyT<KB> *keys, uint64_t n, uint6
     CK(hipMemcpy(L->h_pin + 1, L->d_err, 4, hipMemcpyDeviceToHost));
     const uint64_t n_light = L->h_pin[0];
     if (L->h_pin[1])
-        return QS_ERR_TABLE;
+        L->full = 1;
+    L->inserted += n;
     if (n_heavy)
         *n_heavy = n - n_light;
     if (n_light == 0)
@@ -982,16 +1093,18 @@ extern "C" int qs_learned_query(qs_learned *L, const void *d_keys, uint64_t n, u
         return QS_ERR_PARAM;
     if (n == 0)
         return QS_OK;
-    int rc = qs_query_async(L->sk, d_keys, n, d_counts, 0);
-    if (rc)
-        return rc;
-    const dim3 grid(lgrid((n + 127) / 128, 4096)), block(256);
+    if (n > 0xffffffffull)
+        return QS_ERR_PARAM; // the sketch's query bound
+    const qs_sketch *s = L->sk;
+    const dim3 grid(lgrid((n + 127) / 128, 8192)), block(256);
     if (L->key_bytes == 4)
-        hipLaunchKernelGGL(k_route_query<4>, grid, block, 0, 0, (const uint32_t *)d_keys, n, L->d_pb, L->d_p, L->thr,
-                           L->d_tkeys, L->d_tcnt, L->tlog2, d_counts);
+        hipLaunchKernelGGL(k_learned_query<4>, grid, block, 0, 0, (const uint32_t *)d_keys, n, L->d_pb, L->d_p,
+                           L->thr, L->d_tkeys, L->d_tcnt, L->tlog2, L->full, s->h_desc[0], s->d_low, s->d_high,
+                           s->d_masks, d_counts);
     else
-        hipLaunchKernelGGL(k_route_query<8>, grid, block, 0, 0, (const uint64_t *)d_keys, n, L->d_pb, L->d_p, L->thr,
-                           L->d_tkeys, L->d_tcnt, L->tlog2, d_counts);
+        hipLaunchKernelGGL(k_learned_query<8>, grid, block, 0, 0, (const uint64_t *)d_keys, n, L->d_pb, L->d_p,
+                           L->thr, L->d_tkeys, L->d_tcnt, L->tlog2, L->full, s->h_desc[0], s->d_low, s->d_high,
+                           s->d_masks, d_counts);
     CK(hipGetLastError());
     CK(hipDeviceSynchronize());
     return QS_OK;
@@ -1019,6 +1132,11 @@ extern "C" int qs_learned_table(qs_learned *L, uint64_t *h_keys, uint32_t *h_cou
     }
     *n_used = u;
     return QS_OK;
+}
+
+extern "C" int qs_learned_table_full(const qs_learned *L)
+{
+    return L ? (int)L->full : QS_ERR_PARAM;
 }
 
 extern "C" int qs_learned_memory(const qs_learned *L, uint64_t *table_bytes, uint64_t *mlp_bytes,

@@ -223,3 +223,45 @@ def test_oracle_u64_hash_is_hash_mul_add_in_64_bits():
             if m >> j & 1:
                 want[b * 8 + j // 4] += 1 << (8 * (j % 4))
     assert np.array_equal(o.low(), want)
+
+
+# ---------------------------------------------------------------------------
+# accuracy harness (qsketch_amd/accuracy.py runs on any torch device)
+# ---------------------------------------------------------------------------
+def test_accuracy_exact_counts_equal_the_oracle():
+    """torch's sort + count (the GPU harness) against the oracle's radix sort +
+    reduce-by-key, for u32 keys on both sides of 2^31 and for u64 keys."""
+    import torch
+    from oracle import pyoracle as po
+    from qsketch_amd import accuracy as acc
+    rs = np.random.default_rng(8)
+    k32 = rs.integers(0, 1 << 32, 50000, dtype=np.uint64).astype(np.uint32) % np.uint32(3000) * np.uint32(1431655)
+    gk, gc = acc.exact_counts(torch.from_numpy(k32.view(np.int32).copy()))
+    uk, uc = po.exact_counts(k32, key_bytes=4)
+    g = gk.numpy().view(np.uint32)
+    order = np.argsort(g)  # torch sorts the signed view
+    assert np.array_equal(g[order], uk) and np.array_equal(gc.numpy()[order], uc)
+    k64 = rs.integers(0, 5000, 40000, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    gk, gc = acc.exact_counts(torch.from_numpy(k64.view(np.int64).copy()))
+    uk, uc = po.exact_counts(k64, key_bytes=8)
+    g = gk.numpy().view(np.uint64)
+    order = np.argsort(g)
+    assert np.array_equal(g[order], uk) and np.array_equal(gc.numpy()[order], uc)
+
+
+def test_accuracy_metrics_on_hand_computed_cases():
+    import torch
+    from qsketch_amd import accuracy as acc
+    est = torch.tensor([3, 5, 10, 0], dtype=torch.int32)
+    true = torch.tensor([2, 5, 8, 1])
+    m = acc.metrics(est, true)
+    assert m["are"] == pytest.approx((1 / 2 + 0 + 2 / 8 + 1) / 4)
+    assert m["aae"] == pytest.approx((1 + 0 + 2 + 1) / 4)
+    assert m["mean_est_over_true_pct"] == pytest.approx((150 + 100 + 125 + 0) / 4)
+    assert m["under_estimates"] == 1 and m["max_abs_error"] == 2
+    # estimates are u32 bit patterns in an int32 tensor
+    m = acc.metrics(torch.tensor([-1], dtype=torch.int32), torch.tensor([7]))
+    assert m["aae"] == 4294967295 - 7 and m["under_estimates"] == 0
+    # a key that was never inserted counts in AAE, not in ARE
+    m = acc.metrics(torch.tensor([4, 2], dtype=torch.int32), torch.tensor([2, 0]))
+    assert m["are"] == pytest.approx(1.0) and m["aae"] == pytest.approx(2.0) and m["absent_keys"] == 1
