@@ -239,19 +239,6 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def multi_workload(n_sk, seed=1):
-    """4096 independent streams: lengths log-uniform in [2^10, 2^20], each Zipf
-    with its own exponent (uniform in [0.7, 1.3], on a 0.01 grid so that one
-    sampler table serves many streams), its own seed and its own 2^16-key
-    universe (the rank -> key permutation is keyed by the stream's seed)."""
-    import numpy as np
-    rs = np.random.default_rng(seed)
-    lengths = np.rint(2.0 ** rs.uniform(10, 20, n_sk)).astype(np.int64)
-    alphas = np.round(rs.uniform(0.7, 1.3, n_sk), 2)
-    seeds = np.arange(1, n_sk + 1, dtype=np.int64) * 7919 + seed
-    return lengths, alphas, seeds
-
-
 def bench_multi(a):
     """Batched multi-sketch: one captured graph launch inserts every stream into
     its own sketch, against the same streams inserted by one plain sketch each."""
@@ -259,26 +246,15 @@ def bench_multi(a):
     import torch
     import qsketch_amd as q
 
+    from qsketch_amd import workloads as wl
+
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     K = a.sketches
-    lengths, alphas, seeds = multi_workload(K)
-    offs = np.zeros(K + 1, np.uint64)
-    offs[1:] = np.cumsum(lengths)
+    keys, offs, lengths, alphas, seeds = wl.multi_keys(K)
     N = int(offs[-1])
-    keys = torch.empty(N, dtype=torch.int32, device=dev)
-    samplers = {}
-    for i in range(K):
-        al = float(alphas[i])
-        if al not in samplers:
-            samplers[al] = q.Zipf(al, 1 << 16)
-        samplers[al].keys_u32(int(seeds[i]), int(lengths[i]), out=keys[int(offs[i]):int(offs[i + 1])])
     d_offs = torch.from_numpy(offs.view(np.int64).copy()).to(dev)
-    params = []
-    for i in range(K):
-        n_i = int(lengths[i])
-        params.append(q.make_params(q.number_of_buckets(n_i, 8, 0.5), max(1024, n_i // 256),
-                                    rng_seed=int(seeds[i])))
+    params = wl.multi_params(lengths, seeds)
     S = q.SketchSet(params)
     tune = None
     if not a.no_tune:
