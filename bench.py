@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--wlf", type=float, default=0.5, help="work-load factor of the sizing rule")
     ap.add_argument("--n-high", type=int, default=1 << 18)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the learned-form comparison (ARE at equal memory) of the default line")
     ap.add_argument("--cpu-sample-log2", type=int, default=27,
                     help="log2 of the stream prefix the CPU baseline inserts (2^27: about 20 s of "
                          "thread time on 16 host threads)")
@@ -58,33 +60,97 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(keys_dev, n_low, n_high, sample_log2):
-    """Time the oracle (the restated sequential semantics) on host cores over a
-    bounded prefix of the same stream: insert the prefix, query its distinct keys."""
+def cpu_info() -> dict:
+    """CPU model and clock of this host, read at run time from /proc/cpuinfo."""
+    model, mhz = None, []
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name" and model is None:
+                    model = v.strip()
+                elif k == "cpu MHz":
+                    mhz.append(float(v))
+    except OSError:
+        pass
+    return {"model": model, "mhz_mean": round(sum(mhz) / len(mhz), 1) if mhz else None,
+            "logical_cpus": os.cpu_count()}
+
+
+def host_threads():
+    """Threads for the all-cores variant: every core of the affinity mask, but
+    no more than OMP_NUM_THREADS when that is set (the GPU box gives one GPU's
+    job a share of the host's cores and says so there)."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(env) if env.isdigit() and int(env) > 0 else aff
+    return max(1, min(aff, cap)), aff
+
+
+def _oracle_times(keys, uniq, n_low, n_high, threads, reps):
+    """Median insert and query seconds of the oracle over `reps` runs, each on
+    a cleared sketch whose pages are already touched."""
     import numpy as np
     from oracle import pyoracle as po
+    o = po.OracleSketch(n_low, n_high, 1021, rng_seed=1)
+    ti, tq = [], []
+    for _ in range(reps):
+        o.clear()
+        t0 = time.perf_counter()
+        rc = o.insert(keys, nthreads=threads)
+        t1 = time.perf_counter()
+        o.query(uniq, nthreads=threads)
+        t2 = time.perf_counter()
+        if rc != 0:
+            raise RuntimeError(f"oracle insert failed: {rc}")
+        ti.append(t1 - t0)
+        tq.append(t2 - t1)
+    return float(np.median(ti)), float(np.median(tq))
+
+
+def _rate(n, nq, ti, tq) -> dict:
+    return {"gkeys_s": round((n + nq) / (ti + tq) / 1e9, 5), "insert_mkeys_s": round(n / ti / 1e6, 2),
+            "query_mkeys_s": round(nq / tq / 1e6, 2), "insert_s": round(ti, 4), "query_s": round(tq, 4)}
+
+
+def cpu_baseline(keys_dev, n_low, n_high, sample_log2):
+    """The oracle (the restated sequential semantics; the reference has no CPU
+    sketch) timed on this host's cores:
+      * the measured stream: its first 2^sample_log2 keys inserted, then their
+        distinct keys queried, all threads (bucket-range partitioned, the same
+        state as one thread), median of 3;
+      * cfg1 (2^22 keys, Zipf 1.0 over 2^18, seed 1, WLF 0.5), one thread and
+        all threads, median of 5 each."""
+    import numpy as np
+    import qsketch_amd as q
+    from qsketch_amd import workloads as wl
+    threads, aff = host_threads()
     n = min(1 << sample_log2, keys_dev.numel())
     keys = keys_dev[:n].cpu().numpy().view(np.uint32).copy()
     uniq = np.unique(keys)
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    o = po.OracleSketch(n_low, n_high, 1021, rng_seed=1)
-    o.clear()  # touch every page before timing
-    t0 = time.perf_counter()
-    rc = o.insert(keys, nthreads=threads)
-    t1 = time.perf_counter()
-    o.query(uniq, nthreads=threads)
-    t2 = time.perf_counter()
-    if rc != 0:
-        raise RuntimeError(f"oracle insert failed: {rc}")
+    ti, tq = _oracle_times(keys, uniq, n_low, n_high, threads, 3)
+    main_rate = _rate(n, uniq.size, ti, tq)
+    k1 = wl.plain_keys(**wl.CFG1, device=False)
+    u1 = np.unique(k1)
+    n_low1 = q.number_of_buckets(k1.size, 8, 0.5)
+    one = _rate(k1.size, u1.size, *_oracle_times(k1, u1, n_low1, 1 << 13, 1, 5))
+    allc = _rate(k1.size, u1.size, *_oracle_times(k1, u1, n_low1, 1 << 13, threads, 5))
     return {
-        "value": round((n + uniq.size) / (t2 - t0) / 1e9, 5),
+        "value": main_rate["gkeys_s"],
         "unit": "Gkeys/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first 2^{sample_log2} keys of the same stream inserted ({t1 - t0:.2f} s, "
-                  f"{n / (t1 - t0) / 1e6:.1f} Mkeys/s), then its {uniq.size} distinct keys queried "
-                  f"({t2 - t1:.2f} s); bucket-range partitioned oracle on {threads} threads, "
-                  f"{(t2 - t0) * threads:.1f} thread-seconds",
+        "sample": f"first 2^{sample_log2} keys of the same stream inserted, then their {uniq.size} distinct "
+                  f"keys queried; bucket-range partitioned oracle on {threads} threads, median of 3 "
+                  f"(insert {main_rate['insert_mkeys_s']} Mkeys/s)",
+        "threads_note": f"all-cores variant on {threads} threads: affinity mask {aff} cores, "
+                        f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}",
+        "cpu": cpu_info(),
+        "measured_stream": main_rate,
+        "cfg1": {"workload": "2^22 u32 keys, Zipf 1.0 over 2^18, seed 1; n_low 3,145,721 (WLF 0.5), "
+                             "n_high 2^13; insert all, query every distinct key; median of 5",
+                 "distinct": int(u1.size), "single_core": one, "all_cores": dict(allc, threads=threads)},
     }
 
 
@@ -236,6 +302,11 @@ def main():
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(keys, n_low, a.n_high, a.cpu_sample_log2)
+    if not a.no_accuracy:
+        sk.destroy()
+        del keys, uniq, counts, sk
+        torch.cuda.empty_cache()
+        out["accuracy"] = learned_accuracy(a)
     print(json.dumps(out), flush=True)
 
 
@@ -351,76 +422,44 @@ def bench_multi(a):
     print(json.dumps(out), flush=True)
 
 
-def bench_learned(a):
-    """Learned heavy-hitter form against the plain sketch at equal total memory.
-
-    Stream: 2^28 u64 keys, Zipf 1.1 over a 2^24-key universe; the upper 16 bits
-    of a key hold clip(round(log2(rank + 1)) + N(0, 1.5), 0, 31), the lower 48
-    bits are random (one key per rank).  The network is trained on what a plain
-    sketch estimated for the distinct keys of the first 2^26 items (target
-    log2(1 + estimate)); the threshold admits at most 2^16 of those keys to the
-    exact table.  Errors are measured over every distinct key of the stream
-    against exact counts (sort + reduce-by-key)."""
+def learned_compare(log2n, scales, train_steps, reps, n_high, wlf, single_level=True):
+    """The learned heavy-hitter form against the plain sketch at equal total
+    memory, on the cfg5 stream: 2^log2n u64 keys, Zipf 1.1 over a 2^24-key
+    universe, the upper 16 bits of a key clip(round(log2(rank + 1)) +
+    N(0, 1.5), 0, 31), the lower 48 bits a permutation of the rank.  The
+    network is trained on what a plain sketch estimated for the distinct keys
+    of the first 2^26 items (target log2(1 + estimate)); the threshold admits
+    at most 2^16 of those keys to the exact table.  Errors over every distinct
+    key of the stream against exact counts (sort + reduce-by-key).  Returns
+    (rows per memory scale, training record, config)."""
     import numpy as np
     import torch
     import qsketch_amd as q
     from qsketch_amd import accuracy as acc
+    from qsketch_amd import learned as lf
+    from qsketch_amd import workloads as wl
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    N = 1 << a.log2n
-    U = 1 << 24
+    N = 1 << log2n
     W = min(N, 1 << 26)
-    TOP = 1 << 16
-    TABLE_LOG2 = 19
-    z = q.Zipf(1.1, U)
-    keys = z.keys_u64_noisy(seed=1, n=N, sigma=1.5)
-    n_low0 = q.number_of_buckets(N, 8, a.wlf)
-    n_high = a.n_high
-
-    # 1. training pairs from a plain sketch over the first window
-    p0 = q.make_params(n_low0, n_high, key_bytes=8, rng_seed=1)
-    sk0 = q.Sketch_GPU_Sub_Warp_Level(n_low0, 8, n_high, 32, key_bytes=8, rng_seed=1)
-    sk0.insert(keys[:W])
-    wk = torch.unique(keys[:W])
-    west = sk0.search(wk).to(torch.int64) & 0xFFFFFFFF
-    targets = torch.log2(1.0 + west.to(torch.float32)).contiguous()
-    sk0.destroy()
-    del sk0
-
-    # 2. train (hand-written forward/backward + Adam) on shuffled pairs
-    L0 = q.LearnedSketch(p0, table_log2=TABLE_LOG2, init_seed=7)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(3)
-    B = 4096
-    losses = []
+    TOP, TL = 1 << 16, 19
+    keys = wl.learned_keys(log2n=log2n)
+    n_low0 = q.number_of_buckets(N, 8, wlf)
+    wk, tgt = lf.window_targets(keys[:W], n_low0, n_high)
+    L0 = q.LearnedSketch(q.make_params(1021, n_high, key_bytes=8, rng_seed=1), table_log2=TL, init_seed=7)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for it in range(a.train_steps):
-        idx = torch.randint(0, wk.numel(), (B,), device=dev, generator=gen)
-        want = it % 100 == 0 or it == a.train_steps - 1
-        l = L0.step(wk[idx].contiguous(), targets[idx].contiguous(), lr=2e-3, want_loss=want)
-        if want:
-            losses.append(round(l, 5))
+    losses = lf.train_router(L0, wk, tgt, steps=train_steps)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     params = L0.get_params()
-    # threshold: the largest score set among the window's keys that holds <= TOP keys
-    # (scores tie wherever the network ignores the random low bits)
-    sc = L0.score(wk)
-    srt = torch.sort(sc, descending=True).values
-    thr = float(srt[min(TOP, srt.numel()) - 1])
-    if int((sc >= thr).sum()) > TOP:
-        above = sc[sc > thr]
-        thr = float(above.min()) if above.numel() else float("inf")
-    window_routed = int((sc >= thr).sum())
+    thr = lf.pick_threshold(L0, wk, TOP)
+    window_routed = int((L0.score(wk) >= thr).sum())
+    train_pairs = int(wk.numel())
     L0.destroy()
-    del L0
-
-    # 3. exact counts of the whole stream
+    del wk, tgt
     uk, uc = torch.unique(keys, return_counts=True)
 
-    def time_it(setup, fn, reps):
+    def time_it(setup, fn):
         ts = []
         for _ in range(reps):
             setup()
@@ -432,51 +471,33 @@ def bench_learned(a):
         return float(np.median(ts))
 
     rows = []
-    for scale in [float(x) for x in a.mem_scales.split(",")]:
+    for scale in scales:
         n_low = q.largest_prime_at_most(max(1021, int(n_low0 * scale)))
         plain = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, n_high, 32, key_bytes=8, rng_seed=1)
         mem_plain = plain.memory_bytes()
-        # equal memory: the learned form's sketch gives up the table and network bytes
-        probe = q.LearnedSketch(q.make_params(n_low, n_high, key_bytes=8, rng_seed=1), table_log2=TABLE_LOG2)
-        extra = probe.memory()["table"] + probe.memory()["mlp"]
-        probe.destroy()
-        # (the bucket count stays prime, as the sizing rule's always is)
-        n_low_l = q.largest_prime_at_most(n_low - (extra + 31) // 32)
-        L = q.LearnedSketch(q.make_params(n_low_l, n_high, key_bytes=8, rng_seed=1), table_log2=TABLE_LOG2)
+        n_low_l = lf.equal_memory_n_low(n_low, n_high, TL)
+        L = q.LearnedSketch(q.make_params(n_low_l, n_high, key_bytes=8, rng_seed=1), table_log2=TL)
         L.set_params(params)
         L.threshold = thr
         mem_learned = L.memory()["total"]
-
         plain.insert(keys)  # warm-up (scratch allocation)
         L.insert(keys)
-        t_plain = time_it(plain.clear, lambda: plain.insert(keys), a.steps)
-        t_learned = time_it(L.clear, lambda: L.insert(keys), a.steps)
+        t_plain = time_it(plain.clear, lambda: plain.insert(keys))
+        t_learned = time_it(L.clear, lambda: L.insert(keys))
         L.clear()
         n_heavy = L.insert(keys)
         plain.clear()
         plain.insert(keys)
         est_p = plain.search(uk)
         est_l = L.search(uk)
-        torch.cuda.synchronize()
-        sbuf = torch.empty(N, dtype=torch.float32, device=dev)
-        t_score = time_it(lambda: None, lambda: L.score(keys, out=sbuf), a.steps)
+        tq_plain = time_it(lambda: None, lambda: plain.search(uk, uk.numel(), est_p))
+        tq_learned = time_it(lambda: None, lambda: L.search(uk, uk.numel(), est_l))
+        sbuf = torch.empty(N, dtype=torch.float32, device=keys.device)
+        t_score = time_it(lambda: None, lambda: L.score(keys, out=sbuf))
         del sbuf
-        tq_plain = time_it(lambda: None, lambda: plain.search(uk, uk.numel(), est_p), a.steps)
-        tq_learned = time_it(lambda: None, lambda: L.search(uk, uk.numel(), est_l), a.steps)
         mp = acc.metrics(est_p, uc)
         ml = acc.metrics(est_l, uc)
-        # the reference's single-level sub-warp sketch (m = 8) at the same bytes
-        sl = q.Sketch_GPU_Sub_Warp(q.largest_prime_at_most(mem_plain // 32), 8, key_bytes=8, rng_seed=1)
-        sl.insert(keys)
-        t_sl = time_it(sl.clear, lambda: sl.insert(keys), a.steps)
-        sl.clear()
-        sl.insert(keys)
-        msl = acc.metrics(sl.search(uk), uc)
-        sl_row = {"n": sl.n, "memory": sl.memory_bytes(), "are": msl["are"], "aae": msl["aae"],
-                  "insert_gkeys_s": round(N / t_sl / 1e9, 3)}
-        sl.destroy()
-        tk, _ = L.table()
-        rows.append({
+        row = {
             "mem_scale": scale, "n_low_plain": n_low, "n_low_learned": int(n_low_l),
             "memory_plain": mem_plain, "memory_learned": mem_learned,
             "are_plain": mp["are"], "are_learned": ml["are"],
@@ -489,14 +510,62 @@ def bench_learned(a):
             "insert_throughput_loss": round(1 - t_plain / t_learned, 4),
             "query_gkeys_s_plain": round(uk.numel() / tq_plain / 1e9, 3),
             "query_gkeys_s_learned": round(uk.numel() / tq_learned / 1e9, 3),
-            "heavy_fraction_of_stream": round(n_heavy / N, 4), "table_keys": int(tk.size),
-            "single_level_equal_memory": sl_row,
+            "query_throughput_loss": round(1 - tq_plain / tq_learned, 4),
+            "heavy_fraction_of_stream": round(n_heavy / N, 4), "table_keys": int(L.table()[0].size),
+            "table_full": L.table_full,
             "mlp_forward_gkeys_s": round(N / t_score / 1e9, 3),
             "mlp_forward_tflops": round(N * 2 * (64 * 64 * 2 + 64) / t_score / 1e12, 2),
-        })
+        }
+        if single_level:
+            # the reference's single-level sub-warp sketch (m = 8) at the same bytes
+            sl = q.Sketch_GPU_Sub_Warp(q.largest_prime_at_most(mem_plain // 32), 8, key_bytes=8, rng_seed=1)
+            sl.insert(keys)
+            t_sl = time_it(sl.clear, lambda: sl.insert(keys))
+            sl.clear()
+            sl.insert(keys)
+            msl = acc.metrics(sl.search(uk), uc)
+            row["single_level_equal_memory"] = {"n": sl.n, "memory": sl.memory_bytes(), "are": msl["are"],
+                                                "aae": msl["aae"], "insert_gkeys_s": round(N / t_sl / 1e9, 3)}
+            sl.destroy()
+        rows.append(row)
         plain.destroy()
         L.destroy()
+    training = {"steps": train_steps, "batch": 4096, "seconds": round(t_train, 3),
+                "loss_curve_every_100": [round(x, 5) for x in losses], "threshold": thr,
+                "window_keys_routed": window_routed, "train_pairs": train_pairs}
+    config = {"workload": f"2^{log2n} u64 keys Zipf 1.1 over 2^24, upper 16 bits = clip(round(log2(rank+1)) "
+                          f"+ N(0,1.5), 0, 31); MLP 64-64-64-1 trained on plain-sketch estimates of the first "
+                          f"2^26 items; <= 2^16 window keys over the threshold; table 2^{TL} slots",
+              "keys": N, "distinct": int(uk.numel())}
+    del keys, uk, uc
+    return rows, training, config
 
+
+def learned_accuracy(a) -> dict:
+    """The "ARE at equal memory" half of the headline metric, for the default
+    line: the cfg5 comparison at full memory."""
+    rows, training, config = learned_compare(28, [1.0], a.train_steps, 3, 1 << 18, 0.5, single_level=False)
+    r = rows[0]
+    return {
+        "workload": config["workload"], "keys": config["keys"], "distinct": config["distinct"],
+        "are_plain": r["are_plain"], "are_learned": r["are_learned"], "are_reduction": r["are_reduction"],
+        "aae_plain": r["aae_plain"], "aae_learned": r["aae_learned"],
+        "memory_plain": r["memory_plain"], "memory_learned": r["memory_learned"],
+        "insert_gkeys_s_plain": r["insert_gkeys_s_plain"], "insert_gkeys_s_learned": r["insert_gkeys_s_learned"],
+        "insert_throughput_loss": r["insert_throughput_loss"],
+        "query_gkeys_s_plain": r["query_gkeys_s_plain"], "query_gkeys_s_learned": r["query_gkeys_s_learned"],
+        "query_throughput_loss": r["query_throughput_loss"],
+        "heavy_fraction_of_stream": r["heavy_fraction_of_stream"],
+        "training_seconds": training["seconds"], "train_steps": training["steps"],
+        "final_loss": training["loss_curve_every_100"][-1],
+    }
+
+
+def bench_learned(a):
+    """Learned heavy-hitter form against the plain sketch at equal total memory
+    (several memory budgets); see learned_compare."""
+    rows, training, config = learned_compare(a.log2n, [float(x) for x in a.mem_scales.split(",")],
+                                             a.train_steps, a.steps, a.n_high, a.wlf)
     main_row = rows[0]
     out = {
         "metric": "learned heavy-hitter form: ARE reduction at equal memory and insert Gkeys/s vs plain",
@@ -505,20 +574,14 @@ def bench_learned(a):
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": 1,
-        "ms_per_step": round(N / main_row["insert_gkeys_s_learned"] / 1e6, 3),
+        "ms_per_step": round((1 << a.log2n) / main_row["insert_gkeys_s_learned"] / 1e6, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64 keys; bf16 MLP with fp32 accumulation",
         "data": "synthetic",
-        "config": {
-            "workload": f"2^{a.log2n} u64 keys Zipf 1.1 over 2^24, upper 16 bits = clip(round(log2(rank+1)) "
-                        f"+ N(0,1.5), 0, 31); MLP 64-64-64-1 trained on plain-sketch estimates of the first "
-                        f"2^26 items; <= 2^16 window keys over the threshold; table 2^{TABLE_LOG2} slots",
-            "keys": N, "distinct": int(uk.numel()), "train_pairs": int(wk.numel()),
-        },
-        "training": {"steps": a.train_steps, "batch": B, "seconds": round(t_train, 3),
-                     "loss_curve_every_100": losses, "threshold": thr, "window_keys_routed": window_routed},
+        "config": dict(config, train_pairs=training["train_pairs"]),
+        "training": training,
         "budgets": rows,
     }
     print(json.dumps(out), flush=True)

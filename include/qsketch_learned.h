@@ -7,7 +7,7 @@
  * a threshold are counted exactly in a GPU hash table; every other key goes to
  * an ordinary MSWS sketch (include/qsketch.h), in stream order, so that part
  * keeps the plain sketch's semantics bit for bit.  A query returns the exact
- * count for a key routed to the table and the sketch estimate otherwise.
+ * count for a key held in the table and the sketch estimate otherwise.
  *
  * Training is the caller's loop over minibatches: qs_mlp_step computes the
  * mean-squared-error gradients of a batch (hand-written forward and backward
@@ -25,11 +25,9 @@
  * A full table does not lose keys: a key the table has no slot for (the
  * table is full, or its probe window is) is counted in the sketch instead,
  * with all its occurrences of that insert call and later ones, in stream
- * order.  Every key is thus counted entirely in one part.  A query returns
- * the table count of a key found there, the sketch estimate of a light key
- * or of a heavy key missing from the table once the table has filled, and 0
- * for a heavy key missing from a table that never filled (no insert of it
- * has happened).
+ * order.  Every key is thus counted entirely in one part, and a query
+ * returns the table count of a key held in the table and the sketch estimate
+ * of any other key.
  *
  * Parameter layout (FP32, 8385 values): W1[64][64] (row = hidden unit,
  * column = key bit), b1[64], W2[64][64] (row = layer-2 unit, column = layer-1

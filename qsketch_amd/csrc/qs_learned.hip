@@ -20,8 +20,9 @@
 //               for a stable compaction into the sketch's input
 //   k_reflag    after a routing pass the table could not hold: heavy keys
 //               without a slot go to the sketch instead
-//   k_learned_query  score + answer in one pass: table lookup for heavy
-//               keys, the sketch's bucket read only for the others
+//   k_table_query / k_learned_query  the query: the table count of a key
+//               held in the table, else the sketch estimate (the second
+//               kernel scores keys first and probes only heavy-scored ones)
 #include "qs_internal.h"
 
 #include "../../include/qsketch_learned.h"
@@ -68,15 +69,16 @@ struct qs_learned {
     uint64_t tcap;
     uint64_t *d_tkeys;
     uint32_t *d_tcnt;
-    uint32_t *d_err;   // set by k_route when the table turned a key away
+    uint32_t *d_err;   // [0] set by k_route when the table turned a key away, [1] slots in use
     uint64_t inserted; // keys inserted since create / clear
     uint32_t full;     // the table has turned a key away since create / clear
+    uint32_t used;     // table slots in use after the last insert
     // routing scratch
     uint64_t rcap;     // keys
     uint8_t *d_flag;   // 1 = goes to the sketch
     uint32_t *d_tiles; // light keys per routing tile (+ scan totals)
     void *d_light;
-    uint32_t *h_pin;   // pinned: light total, error flag
+    uint32_t *h_pin;   // pinned: light total, error flag, slots in use
 };
 
 // ---------------------------------------------------------------------------
@@ -516,7 +518,8 @@ __device__ static inline uint32_t t_home(uint64_t key, uint32_t tlog2)
     return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64u - tlog2));
 }
 
-__device__ static inline uint32_t t_insert(uint64_t *tkeys, uint32_t tlog2, uint64_t key)
+// slot of key, claiming a free one if it is absent (*used counts claims)
+__device__ static inline uint32_t t_insert(uint64_t *tkeys, uint32_t tlog2, uint64_t key, uint32_t *used)
 {
     const uint32_t mask = (1u << tlog2) - 1u;
     uint32_t s = t_home(key, tlog2);
@@ -527,6 +530,8 @@ __device__ static inline uint32_t t_insert(uint64_t *tkeys, uint32_t tlog2, uint
         if (cur == T_EMPTY) {
             unsigned long long prev =
                 atomicCAS((unsigned long long *)(tkeys + s), (unsigned long long)T_EMPTY, (unsigned long long)key);
+            if (prev == T_EMPTY)
+                atomicAdd(used, 1u);
             if (prev == T_EMPTY || prev == key)
                 return s;
         }
@@ -548,10 +553,11 @@ __device__ static inline uint32_t t_find(const uint64_t *tkeys, uint32_t tlog2, 
     return T_NONE;
 }
 
+// err[0]: set when the table has no slot for key; err[1]: slots in use
 __device__ static inline void t_add(uint64_t *tkeys, uint32_t *tcnt, uint32_t tlog2, uint64_t key, uint32_t c,
                                     uint32_t *err)
 {
-    const uint32_t s = t_insert(tkeys, tlog2, key);
+    const uint32_t s = t_insert(tkeys, tlog2, key, err + 1);
     if (s == T_NONE)
         atomicOr(err, 1u);
     else
@@ -694,14 +700,18 @@ __global__ __launch_bounds__(ROUTE_THREADS) void k_reflag(const KeyT<KB> *__rest
         tiles[blockIdx.x] = s_light;
 }
 
-// Query of the learned form in one pass: a wavefront scores 32 keys on the
-// matrix cores (lanes 0..31 hold the scores), then each key is answered from
-// the part that counted it.  Heavy keys look up the exact table; the others
-// read their sketch bucket as k_query does, one 16-byte half per lane, with
-// lane l and lane l + 32 sharing key l.  No sketch read is spent on a key the
-// table answers.
-#define LQ_SKETCH 0xffffffffu // answer from the sketch
-#define LQ_ZERO 0xfffffffeu   // heavy, not in a table that never filled: never inserted
+// Queries of the learned form.  A key is counted entirely in one part: only
+// keys the network scores heavy ever enter the table, and a heavy key either
+// holds a slot for all its occurrences or for none (k_reflag).  So a key found
+// in the table is answered with its table count, and every other key with
+// the sketch estimate.  Two kernels give that same answer:
+//   k_table_query    looks every key up in the table first (no network); the
+//                    probe ends at the first empty slot, so it is used while
+//                    the table is at most half full and has never filled
+//   k_learned_query  scores 32 keys per wavefront on the matrix cores and
+//                    probes the table only for heavy-scored keys
+// Both read a key's sketch bucket only when the table does not answer it.
+typedef unsigned int lq_u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ static inline uint32_t lq_sel4(const uint4 &v, uint32_t k)
 {
@@ -710,15 +720,73 @@ __device__ static inline uint32_t lq_sel4(const uint4 &v, uint32_t k)
     return (k & 2u) ? hi : lo;
 }
 
-typedef unsigned int lq_u32x4 __attribute__((ext_vector_type(4)));
+// Sketch estimate of key x by a pair of lanes PAIR apart, one 16-byte half of
+// the bucket each (as k_query); need is the same on both lanes of a pair,
+// and the exchanges stay outside the branches.
+template <int KB, int PAIR>
+__device__ static inline uint32_t lq_sketch(bool need, uint64_t x, uint32_t half, const qs_sdesc &d0,
+                                            const uint32_t *s_masks, const uint32_t *__restrict__ low,
+                                            const uint32_t *__restrict__ high)
+{
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t Mh = 0;
+    if (need) {
+        const KeyT<KB> key = (KeyT<KB>)x;
+        const uint32_t M = s_masks[qs_maskidx_d(key, d0)];
+        const uint32_t b = qs_bucket_d(key, d0);
+        const lq_u32x4 vv = __builtin_nontemporal_load(reinterpret_cast<const lq_u32x4 *>(low + (size_t)b * 8u) + half);
+        v = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+        Mh = half ? (M >> 16) : (M & 0xffffu);
+    }
+    uint32_t lo = 0xffffffffu;
+    for (uint32_t m = Mh; m; m &= m - 1u) {
+        const uint32_t jj = (uint32_t)__builtin_ctz(m);
+        lo = min(lo, (lq_sel4(v, jj >> 2) >> ((jj & 3u) * 8u)) & 0xffu);
+    }
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, PAIR, 64));
+    const uint32_t wo = (uint32_t)__shfl_xor((int)v.w, PAIR, 64);
+    const uint32_t w = half ? v.w : wo;
+    uint32_t hi = 0xffffffffu;
+    if (need && w > QS_LINK_START) {
+        const uint32_t *g = high + (size_t)(w - QS_LINK_START) * 32u + 16u * half;
+        for (uint32_t m = Mh; m; m &= m - 1u)
+            hi = min(hi, g[__builtin_ctz(m)]);
+    }
+    hi = min(hi, (uint32_t)__shfl_xor((int)hi, PAIR, 64));
+    return lo + (w > QS_LINK_START ? hi : 0u);
+}
+
+template <int KB>
+__global__ __launch_bounds__(256) void k_table_query(const KeyT<KB> *__restrict__ keys, uint64_t n,
+                                                     const uint64_t *__restrict__ tkeys,
+                                                     const uint32_t *__restrict__ tcnt, uint32_t tlog2, qs_sdesc d0,
+                                                     const uint32_t *__restrict__ low,
+                                                     const uint32_t *__restrict__ high,
+                                                     const uint32_t *__restrict__ masks, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t s_masks[QS_MAX_MASKS];
+    for (uint32_t e = threadIdx.x; e < d0.n_masks; e += blockDim.x)
+        s_masks[e] = masks[e];
+    __syncthreads();
+    const uint32_t half = threadIdx.x & 1u;
+    const uint64_t stride = (uint64_t)gridDim.x * 128u;
+    for (uint64_t i = (uint64_t)blockIdx.x * 128u + (threadIdx.x >> 1); i < n; i += stride) {
+        const uint64_t x = key64<KB>(keys, i);
+        // the all-ones key is the empty-slot marker and is never in the table
+        const uint32_t s = x != T_EMPTY ? t_find(tkeys, tlog2, x) : T_NONE;
+        const bool need = s == T_NONE;
+        const uint32_t est = lq_sketch<KB, 1>(need, x, half, d0, s_masks, low, high);
+        if (!half)
+            out[i] = need ? est : tcnt[s];
+    }
+}
 
 template <int KB>
 __global__ __launch_bounds__(256) void k_learned_query(const KeyT<KB> *__restrict__ keys, uint64_t n,
                                                        const uint16_t *__restrict__ pb, const float *__restrict__ p,
                                                        float thr, const uint64_t *__restrict__ tkeys,
                                                        const uint32_t *__restrict__ tcnt, uint32_t tlog2,
-                                                       uint32_t table_full, qs_sdesc d0,
-                                                       const uint32_t *__restrict__ low,
+                                                       qs_sdesc d0, const uint32_t *__restrict__ low,
                                                        const uint32_t *__restrict__ high,
                                                        const uint32_t *__restrict__ masks,
                                                        uint32_t *__restrict__ out)
@@ -739,43 +807,15 @@ __global__ __launch_bounds__(256) void k_learned_query(const KeyT<KB> *__restric
         const bool valid = i < n;
         const uint64_t x = xn;
         xn = key_at<KB>(keys, i + nw * 32u, n); // the next tile's key, loaded while this one is scored
-        const float y = mlp_mfma(S, x);
-        uint32_t code = LQ_SKETCH;
-        if (lane < 32u && valid && y >= thr && x != T_EMPTY) {
-            const uint32_t s = t_find(tkeys, tlog2, x);
-            code = s != T_NONE ? s : (table_full ? LQ_SKETCH : LQ_ZERO);
-        }
-        code = (uint32_t)__shfl((int)code, (int)l32, 64); // lane l + 32 takes key l's answer
-        const bool need = valid && code == LQ_SKETCH;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        uint32_t Mh = 0;
-        if (need) {
-            const KeyT<KB> key = (KeyT<KB>)x;
-            const uint32_t M = s_masks[qs_maskidx_d(key, d0)];
-            const uint32_t b = qs_bucket_d(key, d0);
-            const lq_u32x4 vv =
-                __builtin_nontemporal_load(reinterpret_cast<const lq_u32x4 *>(low + (size_t)b * 8u) + half);
-            v = make_uint4(vv[0], vv[1], vv[2], vv[3]);
-            Mh = half ? (M >> 16) : (M & 0xffffu);
-        }
-        uint32_t lo = 0xffffffffu;
-        for (uint32_t m = Mh; m; m &= m - 1u) {
-            const uint32_t jj = (uint32_t)__builtin_ctz(m);
-            lo = min(lo, (lq_sel4(v, jj >> 2) >> ((jj & 3u) * 8u)) & 0xffu);
-        }
-        // both halves of a key take the same path, so the exchanges stay in step
-        lo = min(lo, (uint32_t)__shfl_xor((int)lo, 32, 64));
-        const uint32_t wo = (uint32_t)__shfl_xor((int)v.w, 32, 64);
-        const uint32_t w = half ? v.w : wo;
-        uint32_t hi = 0xffffffffu;
-        if (need && w > QS_LINK_START) {
-            const uint32_t *g = high + (size_t)(w - QS_LINK_START) * 32u + 16u * half;
-            for (uint32_t m = Mh; m; m &= m - 1u)
-                hi = min(hi, g[__builtin_ctz(m)]);
-        }
-        hi = min(hi, (uint32_t)__shfl_xor((int)hi, 32, 64));
+        const float y = mlp_mfma(S, x);         // lanes 0..31: the scores of keys 0..31
+        uint32_t s = T_NONE;
+        if (lane < 32u && valid && y >= thr && x != T_EMPTY)
+            s = t_find(tkeys, tlog2, x);
+        s = (uint32_t)__shfl((int)s, (int)l32, 64); // lane l + 32 takes key l's answer
+        const bool need = valid && s == T_NONE;
+        const uint32_t est = lq_sketch<KB, 32>(need, x, half, d0, s_masks, low, high);
         if (lane < 32u && valid)
-            out[i] = code == LQ_SKETCH ? lo + (w > QS_LINK_START ? hi : 0u) : code == LQ_ZERO ? 0u : tcnt[code];
+            out[i] = need ? est : tcnt[s];
     }
 }
 
@@ -844,7 +884,7 @@ extern "C" int qs_learned_create(const qs_params *p, uint32_t table_log2, uint64
               hipMalloc(&L->d_m, NPAR * 4) == hipSuccess && hipMalloc(&L->d_v, NPAR * 4) == hipSuccess &&
               hipMalloc(&L->d_g, NPAR * 4) == hipSuccess && hipMalloc(&L->d_loss, 4) == hipSuccess &&
               hipMalloc(&L->d_tkeys, L->tcap * 8) == hipSuccess && hipMalloc(&L->d_tcnt, L->tcap * 4) == hipSuccess &&
-              hipMalloc(&L->d_err, 4) == hipSuccess &&
+              hipMalloc(&L->d_err, 8) == hipSuccess &&
               hipHostMalloc(&L->h_pin, 16, hipHostMallocDefault) == hipSuccess;
     if (!ok) {
         qs_learned_destroy(L);
@@ -1016,10 +1056,11 @@ extern "C" int qs_learned_clear(qs_learned *L)
     hipLaunchKernelGGL(k_table_clear, dim3(lgrid((L->tcap + 255) / 256, 2048)), dim3(256), 0, 0, L->d_tkeys,
                        L->d_tcnt, L->tcap);
     CK(hipGetLastError());
-    CK(hipMemset(L->d_err, 0, 4));
+    CK(hipMemset(L->d_err, 0, 8));
     CK(hipDeviceSynchronize());
     L->inserted = 0;
     L->full = 0;
+    L->used = 0;
     return QS_OK;
 }
 
@@ -1060,10 +1101,11 @@ static int learned_insert(qs_learned *L, const KeyT<KB> *keys, uint64_t n, uint6
                        (KeyT<KB> *)L->d_light);
     CK(hipGetLastError());
     CK(hipMemcpy(L->h_pin, L->d_tiles + tiles, 4, hipMemcpyDeviceToHost));
-    CK(hipMemcpy(L->h_pin + 1, L->d_err, 4, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(L->h_pin + 1, L->d_err, 8, hipMemcpyDeviceToHost));
     const uint64_t n_light = L->h_pin[0];
     if (L->h_pin[1])
         L->full = 1;
+    L->used = L->h_pin[2];
     L->inserted += n;
     if (n_heavy)
         *n_heavy = n - n_light;
@@ -1096,15 +1138,27 @@ extern "C" int qs_learned_query(qs_learned *L, const void *d_keys, uint64_t n, u
     if (n > 0xffffffffull)
         return QS_ERR_PARAM; // the sketch's query bound
     const qs_sketch *s = L->sk;
-    const dim3 grid(lgrid((n + 127) / 128, 8192)), block(256);
-    if (L->key_bytes == 4)
-        hipLaunchKernelGGL(k_learned_query<4>, grid, block, 0, 0, (const uint32_t *)d_keys, n, L->d_pb, L->d_p,
-                           L->thr, L->d_tkeys, L->d_tcnt, L->tlog2, L->full, s->h_desc[0], s->d_low, s->d_high,
-                           s->d_masks, d_counts);
-    else
-        hipLaunchKernelGGL(k_learned_query<8>, grid, block, 0, 0, (const uint64_t *)d_keys, n, L->d_pb, L->d_p,
-                           L->thr, L->d_tkeys, L->d_tcnt, L->tlog2, L->full, s->h_desc[0], s->d_low, s->d_high,
-                           s->d_masks, d_counts);
+    const dim3 block(256);
+    if (!L->full && 2ull * L->used <= L->tcap) {
+        // probes of absent keys stay short: look the table up first
+        const dim3 grid(lgrid((2 * n + 255) / 256, 2048));
+        if (L->key_bytes == 4)
+            hipLaunchKernelGGL(k_table_query<4>, grid, block, 0, 0, (const uint32_t *)d_keys, n, L->d_tkeys,
+                               L->d_tcnt, L->tlog2, s->h_desc[0], s->d_low, s->d_high, s->d_masks, d_counts);
+        else
+            hipLaunchKernelGGL(k_table_query<8>, grid, block, 0, 0, (const uint64_t *)d_keys, n, L->d_tkeys,
+                               L->d_tcnt, L->tlog2, s->h_desc[0], s->d_low, s->d_high, s->d_masks, d_counts);
+    } else {
+        const dim3 grid(lgrid((n + 127) / 128, 8192));
+        if (L->key_bytes == 4)
+            hipLaunchKernelGGL(k_learned_query<4>, grid, block, 0, 0, (const uint32_t *)d_keys, n, L->d_pb, L->d_p,
+                               L->thr, L->d_tkeys, L->d_tcnt, L->tlog2, s->h_desc[0], s->d_low, s->d_high,
+                               s->d_masks, d_counts);
+        else
+            hipLaunchKernelGGL(k_learned_query<8>, grid, block, 0, 0, (const uint64_t *)d_keys, n, L->d_pb, L->d_p,
+                               L->thr, L->d_tkeys, L->d_tcnt, L->tlog2, s->h_desc[0], s->d_low, s->d_high,
+                               s->d_masks, d_counts);
+    }
     CK(hipGetLastError());
     CK(hipDeviceSynchronize());
     return QS_OK;
