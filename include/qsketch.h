@@ -136,8 +136,11 @@ int qs_insert_noalloc(qs_sketch *s, const void *d_keys, size_t n, void *stream);
  * (include/qsketch.h:70-71). */
 int qs_export_state(qs_sketch *s, uint32_t *low, uint32_t *high, uint32_t *masks,
                     uint32_t *seed, uint32_t *next);
-/* Replace the whole state (validated: link words in {0} or [1025, 1024+n_high),
- * bytes <= 255, next consistent).  Host pointers, none may be NULL. */
+/* Replace the whole state.  Host pointers, none may be NULL.  QS_ERR_STATE
+ * unless the state is one the insert can reach: 1025 <= next <= 1024 + n_high;
+ * masks of 3 bits below bit 28; link words 0 or in [1025, next), each id of
+ * [1025, next) held by exactly one bucket; byte counters <= 129; the high
+ * rows of row 0 and of ids >= next zero. */
 int qs_import_state(qs_sketch *s, const uint32_t *low, const uint32_t *high,
                     const uint32_t *masks, uint32_t seed, uint32_t next);
 

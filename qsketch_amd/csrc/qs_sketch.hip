@@ -1011,10 +1011,34 @@ static int import_at(qs_sketch *s, uint32_t sid, const uint32_t *low, const uint
     for (uint32_t i = 0; i < d.n_masks; ++i)
         if (__builtin_popcount(masks[i]) != 3 || (masks[i] >> 28) != 0)
             return QS_ERR_STATE;
+    // a state the insert can reach: link words 0 or an id below next, every
+    // id in [1025, next) held by exactly one bucket (the bump allocator of
+    // v3:68-85 hands each out once), byte counters <= 129 (a byte is raised
+    // only while every selected byte is <= 128, v3:160-170), and the high
+    // rows of unallocated ids (and the unused row 0) zero
+    std::vector<uint8_t> seen(next - QS_LINK_START - 1, 0);
     for (uint64_t b = 0; b < d.n_low; ++b) {
-        uint32_t w = low[b * 8 + 7];
-        if (w != 0 && (w <= QS_LINK_START || w >= next))
+        const uint32_t *bk = low + b * 8;
+        const uint32_t w = bk[7];
+        if (w != 0) {
+            if (w <= QS_LINK_START || w >= next || seen[w - QS_LINK_START - 1])
+                return QS_ERR_STATE;
+            seen[w - QS_LINK_START - 1] = 1;
+        }
+        for (int k = 0; k < 7; ++k)
+            for (int j = 0; j < 4; ++j)
+                if (((bk[k] >> (8 * j)) & 0xffu) > QS_LOW_LIMIT + 1u)
+                    return QS_ERR_STATE;
+    }
+    for (uint8_t v : seen)
+        if (!v)
             return QS_ERR_STATE;
+    for (uint64_t r = 0; r < n_high; ++r) {
+        if (r != 0 && r + QS_LINK_START < next)
+            continue;
+        for (int j = 0; j < 32; ++j)
+            if (high[r * 32 + j] != 0)
+                return QS_ERR_STATE;
     }
     CK(hipDeviceSynchronize());
     memcpy(s->h_masks + d.masks_off, masks, sizeof(uint32_t) * d.n_masks);

@@ -134,3 +134,50 @@ def test_empty_and_query_before_insert(gpu):
     g.insert(torch.empty(0, dtype=torch.int32, device="cuda"))
     q = uniform_keys(1000, seed=1)
     assert not to_host_u32(g.search(to_dev(q))).any()
+
+
+def test_import_rejects_unreachable_states(gpu):
+    """qs_import_state accepts the oracle's state and refuses, with
+    QS_ERR_STATE and no change to the sketch, every state the insert cannot
+    reach."""
+    import qsketch_amd as q
+    _print_case(test="import_reject")
+    keys = zipf_keys(1.1, 1 << 12, 1 << 17, seed=4)
+    g, o = make_pair(3067, n_high=4096)
+    assert o.insert(keys) == 0
+    low, high, masks, seed, nxt = o.low(), o.high(), o.masks(), o.seed, o.next
+    assert nxt > 1030
+    g.import_state(low, high, masks, seed, nxt)
+    assert_same_state(g, o, "valid import")
+    linked = np.nonzero(low[7::8])[0]
+    free = np.nonzero(low[7::8] == 0)[0]
+
+    def refused(what, lo=None, hi=None, ma=None, nx=None):
+        with pytest.raises(q.QSketchError) as e:
+            g.import_state(low if lo is None else lo, high if hi is None else hi,
+                           masks if ma is None else ma, seed, nxt if nx is None else nx)
+        assert e.value.code == q.QS_ERR_STATE, what
+        assert_same_state(g, o, f"after refusing {what}")
+
+    lo = low.copy(); lo[free[0] * 8 + 7] = 1024
+    refused("link word 1024", lo=lo)
+    lo = low.copy(); lo[free[0] * 8 + 7] = nxt
+    refused("link word = next", lo=lo)
+    lo = low.copy(); lo[free[0] * 8 + 7] = low[linked[0] * 8 + 7]
+    refused("an id held by two buckets", lo=lo)
+    refused("an id held by no bucket", nx=nxt + 1)
+    lo = low.copy(); lo[free[0] * 8 + 2] = (lo[free[0] * 8 + 2] & ~np.uint32(0xff00)) | np.uint32(130 << 8)
+    refused("byte counter 130", lo=lo)
+    hi = high.copy(); hi[5] = 1
+    refused("counts in high row 0", hi=hi)
+    hi = high.copy(); hi[(nxt - 1024) * 32 + 3] = 1
+    refused("counts in the row of an unallocated id", hi=hi)
+    ma = masks.copy(); ma[7] |= np.uint32(1 << 28)
+    refused("mask bit 28", ma=ma)
+    ma = masks.copy(); ma[9] = np.uint32(0b1111)
+    refused("mask of 4 bits", ma=ma)
+    refused("next below 1025", nx=1024)
+    refused("next beyond the pool", nx=1024 + 4096 + 1)
+    # a byte of 129 is reachable
+    lo = low.copy(); lo[free[0] * 8] = (lo[free[0] * 8] & ~np.uint32(0xff)) | np.uint32(129)
+    g.import_state(lo, high, masks, seed, nxt)
