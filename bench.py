@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the segment-pass shape autotuner")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first copy 1 GiB device to device (a known-traffic kernel for counter calibration)")
+    ap.add_argument("--calib", action="store_true",
+                    help="only measure the ceilings: a streaming copy and random reads of 16/32/64/128 B "
+                         "from a 6 GiB buffer (include/qsketch_calib.h); one JSON line")
     return ap.parse_args()
 
 
@@ -168,6 +171,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if a.calib:
+        return bench_calib(a)
     if a.workload == "multi":
         return bench_multi(a)
     if a.workload == "learned":
@@ -300,6 +305,20 @@ def main():
                   "allocator": "ids assigned in stream order by a sort of the triggering inserts; "
                                "no compare-and-swap loop, so no retries"},
     }
+    # the measured ceilings of the two dominant access patterns, live
+    ceil = ceilings(copy_gib=(4,), gathers=(32,))
+    copy_gbs = ceil["copy"]["4GiB"]["gb_s"]
+    g32 = ceil["gather"]["32B"]["greads_s"]
+    out["roofline_query"].update({
+        "ceiling_random_32B_greads_s": g32,
+        "frac_of_random_32B_ceiling": round(out["query_gkeys_s"] / g32, 4) if out["query_gkeys_s"] else None,
+        "ceiling_note": "2^25 random 32-byte reads from a 6 GiB buffer, 4-byte result each "
+                        "(qs_calib_gather), measured in this run; every random read moves a whole "
+                        "128-byte line (profiles/calibration_copy_gather.md)"})
+    if out["roofline"]["dominant"]:
+        out["roofline"]["dominant"]["frac_of_copy_ceiling"] = round(out["roofline"]["dominant"]["achieved"] / copy_gbs, 4)
+    out["roofline"]["copy_ceiling_gb_s"] = copy_gbs
+    out["roofline"]["frac_of_copy_ceiling"] = round(out["roofline"]["achieved"] / copy_gbs, 4)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(keys, n_low, a.n_high, a.cpu_sample_log2)
     if not a.no_accuracy:
@@ -419,6 +438,63 @@ def bench_multi(a):
         "linked_buckets": st["linked_buckets"],
     }
     g.destroy()
+    print(json.dumps(out), flush=True)
+
+
+def ceilings(reps=5, copy_gib=(1, 4), gathers=(16, 32, 64, 128)):
+    """Ceilings of the access patterns the sketch kernels are made of, with
+    kernels whose traffic is known (qsketch_amd/csrc/qs_calib.hip), timed
+    with events on the stream they run on:
+      * a streaming copy (read + write) of 1 GiB and of 4 GiB;
+      * 2^25 random reads of 16, 32, 64 and 128 bytes from a 6 GiB buffer,
+        each writing a 4-byte result (the query's pattern at 32 bytes)."""
+    import numpy as np
+    import torch
+    import qsketch_amd as q
+    L = q.lib()
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    sp = st.cuda_stream
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            fn()
+            e1.record(st)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return float(np.median(ts))
+
+    out = {"copy": {}, "gather": {}}
+    for gib in copy_gib:
+        src = torch.ones(gib << 28, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        t = timed(lambda: q.check(L.qs_calib_copy(src.data_ptr(), dst.data_ptr(), gib << 30, sp)))
+        out["copy"][f"{gib}GiB"] = {"seconds": round(t, 6), "gb_s": round(2 * (gib << 30) / t / 1e9, 1)}
+        del src, dst
+    if gathers:
+        buf = torch.ones(6 << 28, dtype=torch.int32, device=dev)  # 6 GiB
+        n = 1 << 25
+        res = torch.empty(n, dtype=torch.int32, device=dev)
+        for g in gathers:
+            t = timed(lambda: q.check(L.qs_calib_gather(buf.data_ptr(), 6 << 30, g, n, 7, res.data_ptr(), sp)))
+            out["gather"][f"{g}B"] = {"seconds": round(t, 6), "greads_s": round(n / t / 1e9, 2),
+                                      "gb_s_read": round(n * g / t / 1e9, 1),
+                                      "gb_s_algorithmic": round(n * (g + 4) / t / 1e9, 1)}
+        del buf, res
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_calib(a):
+    """Only the ceilings (see ceilings), as one JSON line."""
+    import torch
+    torch.cuda.set_device(0)
+    out = {"metric": "ceilings of the sketch's access patterns on this GPU", "unit": "GB/s"}
+    out.update(ceilings())
+    out["value"] = out["copy"]["4GiB"]["gb_s"]
     print(json.dumps(out), flush=True)
 
 

@@ -157,6 +157,10 @@ _SIGS = {
     "qs_get_insert_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "qs_autotune_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                      C.POINTER(qs_tune_result)]),
+    # calibration kernels (include/qsketch_calib.h)
+    "qs_calib_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "qs_calib_gather": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p,
+                                  C.c_void_p]),
     # generators (include/qsketch_gen.h)
     "qs_zipf_create": (C.c_int, [C.c_double, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
     "qs_zipf_destroy": (None, [C.c_void_p]),

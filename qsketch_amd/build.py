@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqsketch_amd.so")
-SOURCES = ["qs_sketch.hip", "qs_insert.hip", "qs_sort.hip", "qs_gen.hip", "qs_learned.hip", "qs_single.hip"]
+SOURCES = ["qs_sketch.hip", "qs_insert.hip", "qs_sort.hip", "qs_gen.hip", "qs_learned.hip", "qs_single.hip",
+           "qs_calib.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QS_OFFLOAD_ARCH", "gfx950")
@@ -41,19 +42,22 @@ def _run(cmd):
 
 def build(verbose: bool = False) -> str:
     """Compile every source to an object, then link the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, *SOURCE_FLAGS.get(src, []), "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
+        cmds.append([HIPCC, *FLAGS, *SOURCE_FLAGS.get(src, []), "-c", path, "-o", obj])
         objs.append(obj)
+    for cmd in cmds if verbose else []:
+        print(" ".join(cmd), flush=True)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:  # one hipcc process per source
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
