@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=3000, help="learned: Adam steps of 4096 pairs")
     ap.add_argument("--sketches", type=int, default=4096, help="multi: number of sketches")
     ap.add_argument("--no-tune", action="store_true", help="skip the segment-pass shape autotuner")
+    ap.add_argument("--no-hot-filter", action="store_true",
+                    help="insert every key through the ordered pipeline (same result, for comparison)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first copy 1 GiB device to device (a known-traffic kernel for counter calibration)")
     ap.add_argument("--calib", action="store_true",
@@ -196,6 +198,7 @@ def main():
     n_low = q.number_of_buckets(N, 8, a.wlf)
     sk = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, a.n_high, 32, 1, 1021, rng_seed=1)
     tune = None if a.no_tune else sk.autotune(keys, reps=1)
+    sk.set_hot_filter(not a.no_hot_filter)
     torch.cuda.synchronize()
 
     def step():
@@ -207,9 +210,13 @@ def main():
         step()
     sk.set_timing(True)
     torch.cuda.synchronize()
+    step_s = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
+        ts = time.perf_counter()
         step()
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     tim = sk.timing()
@@ -226,24 +233,31 @@ def main():
     q_achieved = tim["query_keys"] * QUERY_BYTES_PER_KEY / (q_ms * 1e-3) / 1e9 if q_ms > 0 else 0.0
     kern = {k: v for k, v in tim["kernels"].items() if k != "query" and v["launches"]}
     dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
-    # one "launch" of the insert path = the launch sequence of one chunk
-    chunks = max(1, kern.get("seg", {"launches": 1})["launches"])
-    keys_per_launch = ins_keys / chunks
+    # one "launch" of the insert path = the kernel sequence of one insert call
+    # of N keys (the hot-key filter, then the ordered pipeline on the prefix
+    # and on the keys the filter leaves)
+    keys_per_launch = N
+    path_ms = ins_ms / a.steps
     default = all(getattr(a, k) == v for k, v in DEFAULT_WORKLOAD.items())
     traffic = round(INSERT_TRAFFIC_BYTES_PER_KEY * keys_per_launch) if default else None
-    # the dominant kernel on its own: the radix scatter moves one 8-byte item
-    # in and out per key and pass (the first pass reads the 4-byte key), and
-    # every pass but the last writes the next pass's one-byte digit
+    # the radix scatter on its own: it moves one 8-byte item in and out per
+    # sorted key and pass (the first pass reads the 4-byte key), and every pass
+    # but the last writes the next pass's one-byte digit; the filter's hot
+    # keys are never sorted
     sd = kern.get("sort_down")
     dom_obj = None
+    hot_per_step = st["hot_keys"]  # counted since the last clear: the last step's insert
+    sorted_items = N - hot_per_step
     if sd:
-        passes_per_chunk = sd["launches"] / chunks
-        item_bytes = (4 + 8 + (passes_per_chunk - 1) * 16 + (passes_per_chunk - 1)) / passes_per_chunk
-        sd_avg = sd["ms"] / sd["launches"]
-        dom_obj = {"kernel": "k_bs_down (radix scatter)", "avg_launch_ms": round(sd_avg, 4),
-                   "algorithmic_bytes_per_launch": round(item_bytes * keys_per_launch),
-                   "achieved": round(item_bytes * keys_per_launch / (sd_avg * 1e-3) / 1e9, 1),
-                   "frac": round(item_bytes * keys_per_launch / (sd_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        chunks = max(1, kern.get("seg", {"launches": 1})["launches"])
+        passes = sd["launches"] / chunks
+        item_bytes = 4 + 8 + (passes - 1) * 16 + (passes - 1)
+        sd_ms = sd["ms"] / a.steps
+        dom_obj = {"kernel": "k_bs_down (radix scatter)", "ms_per_step": round(sd_ms, 4),
+                   "launches_per_step": sd["launches"] / a.steps, "sorted_keys_per_step": round(sorted_items),
+                   "algorithmic_bytes_per_step": round(item_bytes * sorted_items),
+                   "achieved": round(item_bytes * sorted_items / (sd_ms * 1e-3) / 1e9, 1),
+                   "frac": round(item_bytes * sorted_items / (sd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     out = {
         "metric": "insert & query Gkeys/s (% HBM roofline) at Zipf 1.1; ARE at equal memory",
         "value": round(value, 4),
@@ -262,13 +276,17 @@ def main():
                         f"then query all {n_q} distinct keys",
             "n_low": n_low, "n_high": a.n_high, "wlf": a.wlf, "keys": N, "distinct": n_q,
         },
+        "step_ms": [round(x * 1e3, 3) for x in step_s],
+        "hot_filter": not a.no_hot_filter,
         "insert_gkeys_s": round(ins_keys / (ins_ms * 1e-3) / 1e9, 3) if ins_ms > 0 else None,
         "query_gkeys_s": round(tim["query_keys"] / (q_ms * 1e-3) / 1e9, 3) if q_ms > 0 else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": "insert path: per radix pass k_bs_up (first pass) or k_bs_up_dig, "
-                      "k_hist_scan_local, k_hist_scan_groups, k_bs_down; then k_segw, k_ordered, k_trig_keys + pair sort, k_assign, k_tail, "
-                      "k_clear_long, k_next, k_reset",
+            "kernel": "insert path: hot-key filter (sample sort, k_hot_cand, k_hot_build, k_hot_count, "
+                      "k_scan_*, k_hot_compact, k_hot_flush); then on the prefix and on the cold keys, per "
+                      "radix pass k_bs_up (first pass) or k_bs_up_dig, k_hist_scan_local, "
+                      "k_hist_scan_groups, k_bs_down; then k_segw, k_ordered, k_trig_keys + pair sort, "
+                      "k_assign, k_tail, k_clear_long, k_next, k_reset",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -278,13 +296,14 @@ def main():
                               "counter runs of this workload (calibrated by a 1 GiB copy) in "
                               "profiles/insert_query_plain_zipf1.1_2e28_traffic.md, times the keys "
                               "of one launch" if traffic else None,
-            "launch": "the launch sequence of one insert chunk (radix passes, k_segw, k_ordered, "
-                      "k_assign, k_tail, ...; a 2^28-key insert is one chunk), timed with HIP events "
-                      "around each kernel on its stream",
+            "launch": "the kernel sequence of one insert call of all N keys (hot-key filter; radix "
+                      "passes, k_segw, k_ordered, k_assign, k_tail, ... on the prefix and on the keys "
+                      "the filter leaves), timed with HIP events around each kernel on its stream",
             "bytes_per_key": INSERT_BYTES_PER_KEY,
             "keys_per_launch": round(keys_per_launch),
-            "avg_launch_ms": round(ins_ms / chunks, 4),
-            "path_ms_per_step": round(ins_ms / a.steps, 4),
+            "avg_launch_ms": round(path_ms, 4),
+            "path_ms_per_step": round(path_ms, 4),
+            "hot_filtered_keys_per_step": round(hot_per_step),
             "dominant_kernel": dom,
             "dominant": dom_obj,
             "kernel_ms_per_step": {k: round(v["ms"] / a.steps, 4) for k, v in kern.items()},

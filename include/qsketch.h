@@ -111,6 +111,16 @@ int qs_query_async(qs_sketch *s, const void *d_keys, size_t n, uint32_t *d_count
                    void *stream);
 int qs_sync(qs_sketch *s);
 
+/* Hot-key filter of the blocking qs_insert (build addition; on by default).
+ * A call of at least 2^26 32-bit keys into a plain sketch whose first 2^20
+ * keys show skew (keys occurring 8 or more times there covering 30% or more
+ * of them) inserts its first 2^25 keys the ordinary way; after that, the
+ * inserts of those frequent keys whose bucket is now linked are counted and
+ * added to their high rows at once (such inserts commute), and only the other
+ * keys go through the ordered insert.  The result is the same state, bit for
+ * bit.  enable = 0 turns it off for this handle. */
+int qs_set_hot_filter(qs_sketch *s, int enable);
+
 /* Allocation-free insert.  No counterpart in the reference: it is for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same result as
@@ -162,6 +172,7 @@ typedef struct qs_stats {
     uint64_t long_segments;             /* bucket segments handled by the tail pass */
     uint32_t n_sketches;
     uint32_t pool_error;                /* 1 once a high pool overflowed */
+    uint64_t hot_keys;                  /* inserts counted by the hot-key filter */
 } qs_stats;
 int qs_get_stats(qs_sketch *s, qs_stats *st);
 
@@ -179,7 +190,8 @@ int qs_get_stats(qs_sketch *s, qs_stats *st);
 #define QS_TK_ASSIGN    5  /* allocation ids in stream order                 */
 #define QS_TK_TAIL      6  /* long (hot) segments: windowed high adds        */
 #define QS_TK_QUERY     7  /* query kernel                                   */
-#define QS_TIMING_KERNELS 8
+#define QS_TK_HOT       8  /* hot-key filter (qs_set_hot_filter)             */
+#define QS_TIMING_KERNELS 9
 typedef struct qs_timing {
     double   insert_kernel_ms;      /* sum of kernel_ms[0..6] */
     uint64_t insert_kernel_launches;

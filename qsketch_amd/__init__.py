@@ -64,11 +64,11 @@ class qs_stats(C.Structure):
         ("chunks", C.c_uint64), ("chunks_with_fixup", C.c_uint64), ("fixup_keys", C.c_uint64),
         ("linked_buckets", C.c_uint64), ("next_id", C.c_uint32), ("seed", C.c_uint32),
         ("ordered_segments", C.c_uint64), ("long_segments", C.c_uint64),
-        ("n_sketches", C.c_uint32), ("pool_error", C.c_uint32),
+        ("n_sketches", C.c_uint32), ("pool_error", C.c_uint32), ("hot_keys", C.c_uint64),
     ]
 
 
-TIMING_KERNELS = ["sort_up", "sort_scan", "sort_down", "seg", "ordered", "assign", "tail", "query"]
+TIMING_KERNELS = ["sort_up", "sort_scan", "sort_down", "seg", "ordered", "assign", "tail", "query", "hot"]
 
 
 class qs_timing(C.Structure):
@@ -76,7 +76,7 @@ class qs_timing(C.Structure):
         ("insert_kernel_ms", C.c_double), ("insert_kernel_launches", C.c_uint64),
         ("insert_keys", C.c_uint64), ("query_kernel_ms", C.c_double),
         ("query_kernel_launches", C.c_uint64), ("query_keys", C.c_uint64),
-        ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8),
+        ("kernel_ms", C.c_double * 9), ("kernel_launches", C.c_uint64 * 9),
     ]
 
 
@@ -103,6 +103,7 @@ _SIGS = {
     "qs_query_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     "qs_sync": (C.c_int, [C.c_void_p]),
     "qs_insert_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "qs_set_hot_filter": (C.c_int, [C.c_void_p, C.c_int]),
     "qs_export_state": (C.c_int, [C.c_void_p] + [C.c_void_p] * 5),
     "qs_import_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "qs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(qs_stats)]),
@@ -391,6 +392,10 @@ class Sketch_GPU_Sub_Warp_Level:
     def set_shape(self, width: int, items_per_lane: int) -> None:
         """Segment-pass shape: sub-wavefront width x items per lane (new sketches: (64, 4); (0, 0) = the 64-lane one-item kernel)."""
         check(_lib().qs_set_insert_shape(self._h, int(width), int(items_per_lane)), "qs_set_insert_shape")
+
+    def set_hot_filter(self, enable: bool) -> None:
+        """Hot-key filter of the blocking insert (on by default; same result either way)."""
+        check(_lib().qs_set_hot_filter(self._h, 1 if enable else 0), "qs_set_hot_filter")
 
     def shape(self):
         w, i = C.c_uint32(), C.c_uint32()

@@ -312,7 +312,15 @@ struct qs_sketch {
     int graphs;             // live captured graphs (they hold the scratch pointers)
     uint32_t seg_w;         // segment kernel shape: group width (0 = the 64-lane k_seg)
     uint32_t seg_ipl;       //   and items per lane
+    int hot_filter;         // blocking inserts may use the hot-key filter (qs_hot.hip)
+    void *hot;              // its scratch
+    uint64_t hot_keys;      // inserts counted by the filter since create / clear
 };
+
+// hot-key filter (qs_hot.hip): blocking insert of 32-bit keys into a plain
+// sketch; *used = 0 and nothing inserted when the call does not qualify
+int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stream, int *used);
+void qs_hot_free(qs_sketch *s);
 
 // Stable LSD radix sort of (key, value) uint32 pairs, 8 bits per pass, over the
 // low key_bits bits.  Input in keys[0]/vals[0]; returns the index (0 or 1) of

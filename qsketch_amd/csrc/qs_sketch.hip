@@ -351,6 +351,7 @@ extern "C" void qs_destroy(qs_sketch *s)
         s->tim = nullptr;
     }
     qs_free_scratch(s);
+    qs_hot_free(s);
     hipFree(s->d_low);
     hipFree(s->d_high);
     hipFree(s->d_masks);
@@ -412,6 +413,7 @@ static int create_set(const qs_params *ps, uint32_t nsk, qs_sketch **out)
     // pick for both the 2^28-key Zipf stream and the 4096-sketch batch)
     s->seg_w = 64;
     s->seg_ipl = 4;
+    s->hot_filter = 1;
     s->max_stream = 1ull << pk.ib;
     if (hipGetDevice(&s->device) != hipSuccess) {
         free(s);
@@ -541,6 +543,7 @@ extern "C" int qs_clear(qs_sketch *s)
     s->keys_inserted = 0;
     s->chunks = 0;
     s->sticky_err = 0;
+    s->hot_keys = 0;
     return QS_OK;
 }
 
@@ -566,8 +569,22 @@ static int insert_single(qs_sketch *s, const void *d_keys, size_t n, hipStream_t
         return s->sticky_err;
     if (n == 0)
         return QS_OK;
+    if (blocking && s->hot_filter) {
+        int used = 0;
+        const int rc = qs_insert_hot(s, d_keys, n, stream, &used);
+        if (rc || used)
+            return rc;
+    }
     const uint64_t offs[2] = {0, (uint64_t)n};
     return qs_insert_set(s, d_keys, offs, nullptr, stream, blocking, false);
+}
+
+extern "C" int qs_set_hot_filter(qs_sketch *s, int enable)
+{
+    if (!s)
+        return QS_ERR_PARAM;
+    s->hot_filter = enable != 0;
+    return QS_OK;
 }
 
 extern "C" int qs_insert(qs_sketch *s, const void *d_keys, size_t n)
@@ -1138,5 +1155,6 @@ extern "C" int qs_get_stats(qs_sketch *s, qs_stats *st)
     st->seed = s->h_desc[0].seed;
     st->n_sketches = s->nsk;
     st->pool_error = (h.err & QS_EF_POOL) ? 1u : 0u;
+    st->hot_keys = s->hot_keys;
     return QS_OK;
 }

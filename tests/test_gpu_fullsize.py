@@ -54,4 +54,42 @@ def test_full_size_state_and_estimates(gpu, alpha, log2n, log2u, n_high):
     est = g.search(uq).cpu().numpy().view(np.uint32)
     ref = o.query(uq.cpu().numpy().view(np.uint32), nthreads=_threads())
     assert np.array_equal(est, ref)
-    print(f"  linked buckets {o.next - 1025}, distinct keys {uq.numel()}", flush=True)
+    print(f"  linked buckets {o.next - 1025}, distinct keys {uq.numel()}, "
+          f"hot-filtered inserts {g.stats()['hot_keys']}", flush=True)
+    if alpha >= 1.1:
+        assert g.stats()["hot_keys"] > N // 2  # the skewed streams took the hot-key filter
+
+
+@pytest.mark.parametrize("alpha,log2u", [(1.1, 22), (1.5, 20)])
+def test_hot_filter_same_state_as_ordered_insert(gpu, alpha, log2u):
+    """The hot-key filter (2^26 keys: the shortest call it takes) against the
+    ordinary insert of the same keys and against the oracle; then a second
+    call into the same sketch, whose hot buckets are linked from the start."""
+    import torch
+    import qsketch_amd as q
+    print(f"CASE hot_filter alpha={alpha} u=2^{log2u}", flush=True)
+    N = 1 << 26
+    z = q.Zipf(alpha, 1 << log2u, device=False)
+    k1, k2 = z.keys_u32_host(3, N), z.keys_u32_host(4, N)
+    n_low = q.number_of_buckets(N, 8, 0.5)
+    a = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, 1 << 16, 32, rng_seed=2)
+    b = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, 1 << 16, 32, rng_seed=2)
+    b.set_hot_filter(False)
+    o = po.OracleSketch(n_low, 1 << 16, 1021, rng_seed=2)
+    for k in (k1, k2):
+        d = torch.from_numpy(k.view(np.int32)).cuda()
+        a.insert(d)
+        b.insert(d)
+        assert o.insert(k, nthreads=_threads()) == 0
+    assert a.stats()["hot_keys"] > N // 2, a.stats()["hot_keys"]
+    assert b.stats()["hot_keys"] == 0
+    assert a.stats()["keys_inserted"] == 2 * N
+    sa, sb = a.export_state(), b.export_state()
+    for f in ("low", "high", "masks", "seed", "next"):
+        assert np.array_equal(sa[f], sb[f]), f
+    assert sa["next"] == o.next
+    assert np.array_equal(sa["high"], o.high(copy=False))
+    assert np.array_equal(sa["low"], o.low(copy=False))
+    uq = np.unique(np.concatenate([k1, k2]))
+    est = a.search(torch.from_numpy(uq.view(np.int32)).cuda()).cpu().numpy().view(np.uint32)
+    assert np.array_equal(est, o.query(uq, nthreads=_threads()))
