@@ -426,6 +426,7 @@ def bench_multi(a):
     st = S.stats()
 
     dt = float(np.median(t_set))
+    kern_ms = sum(v["ms"] for k, v in tim["kernels"].items() if k != "query" and v["launches"])
     out = {
         "metric": "batched multi-sketch insert Gkeys/s (one captured launch) vs sequential plain inserts",
         "value": round(N / dt / 1e9, 4),
@@ -454,7 +455,17 @@ def bench_multi(a):
         "shape": list(S.shape()),
         "tuner": tune,
         "kernel_ms_per_step": {k: round(v["ms"], 4) for k, v in tim["kernels"].items() if v["launches"]},
+        "kernel_ms_sum": round(kern_ms, 4),
+        "launch_overhead_ms": round(dt * 1e3 - kern_ms, 4),
         "linked_buckets": st["linked_buckets"],
+        "roofline": {
+            "bound": "hbm", "kernel": "the captured insert launch sequence of the set (radix passes, "
+                                      "k_segw, k_ordered, k_assign, k_tail, ...)",
+            "bytes_per_key": INSERT_BYTES_PER_KEY,
+            "achieved": round(N * INSERT_BYTES_PER_KEY / dt / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(N * INSERT_BYTES_PER_KEY / dt / 1e9 / HBM_PEAK_GBS, 4),
+            "avg_launch_ms": round(dt * 1e3, 4), "traffic": None,
+        },
     }
     g.destroy()
     print(json.dumps(out), flush=True)
