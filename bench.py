@@ -47,9 +47,10 @@ def parse():
     ap.add_argument("--cpu-sample-log2", type=int, default=27,
                     help="log2 of the stream prefix the CPU baseline inserts (2^27: about 20 s of "
                          "thread time on 16 host threads)")
-    ap.add_argument("--workload", choices=["plain", "multi", "learned"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "multi", "learned", "sf"], default="plain",
                     help="plain: one sketch (default); multi: 4096 independent sketches in one batch; "
-                         "learned: MLP-routed heavy hitters + sketch vs plain sketch at equal memory")
+                         "learned: MLP-routed heavy hitters + sketch vs plain sketch at equal memory; "
+                         "sf: the SF sketch with split counters (m = 8, 3 splits)")
     ap.add_argument("--mem-scales", type=str, default="1,0.25,0.0625",
                     help="learned: sketch memory budgets as fractions of the sizing rule's")
     ap.add_argument("--train-steps", type=int, default=3000, help="learned: Adam steps of 4096 pairs")
@@ -179,6 +180,8 @@ def main():
         return bench_multi(a)
     if a.workload == "learned":
         return bench_learned(a)
+    if a.workload == "sf":
+        return bench_sf(a)
     import torch
     import qsketch_amd as q
 
@@ -516,6 +519,75 @@ def ceilings(reps=5, copy_gib=(1, 4), gathers=(16, 32, 64, 128)):
         del buf, res
     torch.cuda.empty_cache()
     return out
+
+
+def bench_sf(a):
+    """The SF sketch (device_sketch_v42.h) on the measured stream: 2^log2n u32
+    keys Zipf(alpha) over 2^log2u, m = 8 counters per bucket split 3 ways (the
+    reference driver's choice, src/main.cu:484-485), n by the sizing rule at
+    WLF 0.5.  One step = clear, insert (with the update of the largest
+    splits), query every distinct key.  Beside it, the oracle on one host
+    thread over the first 2^22 keys into a table sized for them."""
+    import numpy as np
+    import torch
+    import qsketch_amd as q
+    from oracle import pyoracle as po
+    from qsketch_amd import workloads as wl
+    torch.cuda.set_device(0)
+    N = 1 << a.log2n
+    keys = wl.plain_keys(a.alpha, a.log2u, a.log2n)
+    uniq = torch.unique(keys)
+    counts = torch.empty_like(uniq)
+    n = q.number_of_buckets(N, 8, a.wlf)
+    sk = q.Sketch_GPU_SF_Host_Sub_Warp(n, 8, 3, 1, 1021, rng_seed=1)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    for _ in range(a.warmup):
+        sk.clear()
+        sk.insert(keys)
+        sk.search(uniq, uniq.numel(), counts)
+    ti, tq, tc = [], [], []
+    for _ in range(a.steps):
+        tc.append(timed(sk.clear))
+        ti.append(timed(lambda: sk.insert(keys)))
+        tq.append(timed(lambda: sk.search(uniq, uniq.numel(), counts)))
+    t_i, t_q, t_c = (float(np.median(x)) for x in (ti, tq, tc))
+    upd_bytes = n * 8 * 4 * 4  # the update streams 3 splits and writes the largest, per counter
+    # CPU: the oracle, one thread, on a bounded prefix with a table sized for it
+    hk = keys[:1 << 22].cpu().numpy().view(np.uint32).copy()
+    hu = np.unique(hk)
+    o = po.OracleSF(q.number_of_buckets(hk.size, 8, a.wlf), 8, 3, 1021, rng_seed=1)
+    t0 = time.perf_counter()
+    o.insert(hk)
+    t1 = time.perf_counter()
+    o.query(hu)
+    t2 = time.perf_counter()
+    out = {
+        "metric": "SF sketch (split counters) insert + query Gkeys/s",
+        "value": round((N + uniq.numel()) / (t_i + t_q) / 1e9, 4),
+        "unit": "Gkeys/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round((t_c + t_i + t_q) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"SF sketch m=8 times=3, n={n} (WLF {a.wlf}); insert 2^{a.log2n} u32 keys "
+                               f"{'zipf' + str(a.alpha) if a.alpha > 0 else 'uniform'} over 2^{a.log2u}, "
+                               f"then query the {uniq.numel()} distinct keys",
+                   "keys": N, "distinct": int(uniq.numel()), "memory_bytes": sk.memory_bytes()},
+        "insert_gkeys_s": round(N / t_i / 1e9, 3), "insert_ms": round(t_i * 1e3, 3),
+        "query_gkeys_s": round(uniq.numel() / t_q / 1e9, 3), "query_ms": round(t_q * 1e3, 3),
+        "update_bytes_per_insert_call": upd_bytes,
+        "cpu_baseline": {"value": round((hk.size + hu.size) / (t2 - t0) / 1e9, 5), "unit": "Gkeys/s", "cores": 1,
+                         "kind": "port",
+                         "sample": f"first 2^22 keys of the same stream into a table sized for them, then their "
+                                   f"{hu.size} distinct keys queried; oracle on one thread (insert "
+                                   f"{hk.size / (t1 - t0) / 1e6:.1f} Mkeys/s with its update)"},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_calib(a):

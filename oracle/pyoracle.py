@@ -80,6 +80,13 @@ def lib():
         L.orq_sl_insert.argtypes = [C.POINTER(_Sl), vp, sz]
         L.orq_sl_query.argtypes = [C.POINTER(_Sl), vp, sz, C.POINTER(u32)]
         L.orq_exact_counts.restype = sz
+        L.orq_sf_create.argtypes = [C.POINTER(_Sf), u64, u32, u32, u32, u32, u32, u64]
+        L.orq_sf_create.restype = C.c_int
+        L.orq_sf_destroy.argtypes = [C.POINTER(_Sf)]
+        L.orq_sf_insert.argtypes = [C.POINTER(_Sf), vp, sz]
+        L.orq_sf_query.argtypes = [C.POINTER(_Sf), vp, sz, C.POINTER(u32)]
+        L.orq_generate_hashmask_idx.argtypes = [C.POINTER(_Minstd), u32, u32, u32, u32, C.POINTER(C.c_uint8),
+                                                C.POINTER(u32)]
         _lib = L
     return _lib
 
@@ -244,6 +251,59 @@ class OracleSingleLevel:
 
     def table(self) -> np.ndarray:
         return np.ctypeslib.as_array(self.s.table, shape=(self.s.n * self.s.m,)).copy()
+
+
+class _Sf(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("m", C.c_uint32), ("times", C.c_uint32), ("n_masks", C.c_uint32),
+                ("key_bytes", C.c_uint32), ("seed", C.c_uint64 * 3), ("masks", C.POINTER(C.c_uint32)),
+                ("index", C.POINTER(C.c_uint8)), ("split", C.POINTER(C.c_uint32)),
+                ("search", C.POINTER(C.c_uint32))]
+
+
+class OracleSF:
+    """SF sketch (device_sketch_v42.h:207-480): n buckets of m counters, each
+    split into `times` sub-counters; the counter a query reads is the largest
+    of its splits, and the estimate the smallest of the mask's counters."""
+
+    def __init__(self, n, m=8, times=3, n_masks=1021, seed_sz=1, key_bytes=4, rng_seed=1):
+        self.s = _Sf()
+        self.key_bytes = key_bytes
+        rc = lib().orq_sf_create(C.byref(self.s), int(n), int(m), int(times), int(n_masks), int(seed_sz),
+                                 int(key_bytes), int(rng_seed))
+        if rc:
+            raise RuntimeError(f"orq_sf_create failed: {rc}")
+
+    def __del__(self):
+        if getattr(self, "s", None) is not None and self.s.split:
+            lib().orq_sf_destroy(C.byref(self.s))
+
+    def _keys(self, keys):
+        return np.ascontiguousarray(keys, dtype=np.uint32 if self.key_bytes == 4 else np.uint64)
+
+    def insert(self, keys) -> None:
+        k = self._keys(keys)
+        lib().orq_sf_insert(C.byref(self.s), k.ctypes.data, k.size)
+
+    def query(self, keys) -> np.ndarray:
+        k = self._keys(keys)
+        out = np.zeros(k.size, dtype=np.uint32)
+        lib().orq_sf_query(C.byref(self.s), k.ctypes.data, k.size, _p32(out))
+        return out
+
+    def seeds(self):
+        return [int(x) for x in self.s.seed]
+
+    def masks(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.s.masks, shape=(self.s.n_masks,)).copy()
+
+    def index(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.s.index, shape=(self.s.n_masks * 3,)).copy()
+
+    def split(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.s.split, shape=(self.s.n * self.s.m * self.s.times,)).copy()
+
+    def search(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.s.search, shape=(self.s.n * self.s.m,)).copy()
 
 
 def exact_counts(keys, key_bytes=4):

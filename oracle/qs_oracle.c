@@ -38,6 +38,13 @@
  *                          pinned: brute force against a naive Python model,
  *                          the conservation invariant sum = 3N, and the
  *                          textbook count-min bound (estimate >= true count)
+ *   orq_sf_*               device_sketch_v42.h:207-480 (SF sketch: split
+ *                          counters, max over the splits, min over the mask)
+ *                          pinned: brute force against a naive Python model,
+ *                          the invariants sum(split) = 3N and search = max of
+ *                          the splits, estimate >= true count, and at
+ *                          times = 1 equality with orq_sl (the same seeds and
+ *                          masks: it is then the single-level sketch)
  *   orq_exact_counts       host_sketch.h:15-63 (std::unordered_map counter)
  *                          pinned: equality with a Python dict counter
  *
@@ -217,6 +224,26 @@ uint64_t orq_number_of_buckets(uint64_t insert_keys, uint32_t m, double factor)
 /* ------------------------------------------------------------------------ */
 /* masks: lib/hashmask.cu:5-82                                              */
 /* ------------------------------------------------------------------------ */
+void orq_generate_hashmask_idx(orq_minstd *eng, uint32_t n, uint32_t m_bits, uint32_t ones, uint32_t padding,
+                               uint8_t *out, uint32_t *index)
+{
+    uint32_t sz = (m_bits + 7) / 8;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint8_t *hm = out + (size_t)i * sz;
+        uint32_t j = 0;
+        while (j < ones) {
+            uint32_t id = orq_uniform_u32(eng, 0, m_bits - padding - 1);
+            uint8_t bit = (uint8_t)(1u << (id % 8));
+            if ((hm[id / 8] & bit) == 0) {
+                hm[id / 8] |= bit;
+                if (index)
+                    index[(size_t)i * ones + j] = id; /* lib/hashmask.cu:69-71 */
+                ++j;
+            }
+        }
+    }
+}
+
 void orq_generate_hashmask(orq_minstd *eng, uint32_t n, uint32_t m_bits,
                            uint32_t ones, uint32_t padding, uint8_t *out)
 {
@@ -598,6 +625,117 @@ static void radix_sort_u64(uint64_t *a, size_t n, int nbytes)
         memcpy(a, tmp, sizeof(uint64_t) * n);
     }
     free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* SF sketch (device_sketch_v42.h:207-480)                                   */
+/* ------------------------------------------------------------------------ */
+int orq_sf_create(orq_sf *s, uint64_t n, uint32_t m, uint32_t times, uint32_t n_masks, uint32_t seed_sz,
+                  uint32_t key_bytes, uint64_t rng_seed)
+{
+    memset(s, 0, sizeof(*s));
+    if (n == 0 || (m != 8 && m != 16 && m != 32) || times == 0 || n_masks == 0 || seed_sz == 0 ||
+        (key_bytes != 4 && key_bytes != 8))
+        return ORQ_ERR_ARG;
+    int8_t buf[128];
+    orq_minstd seed_eng, mask_eng;
+    orq_seed_buffer(rng_seed, 0, buf);
+    orq_minstd_seed_bytes(&seed_eng, buf, 128);
+    orq_seed_buffer(rng_seed, 1, buf);
+    orq_minstd_seed_bytes(&mask_eng, buf, 128);
+    /* ds.resize(1 + HASH_MASK_ONES, ss) (v42:263): (1 + 3) * ss draws in a row;
+     * insert uses groups 0..2 (seed + j * seed_sz, v42:346), group 0 also for
+     * the bucket (v42:334) */
+    for (uint32_t g = 0; g < 4; ++g) {
+        uint64_t sum = 0;
+        for (uint32_t i = 0; i < seed_sz; ++i)
+            sum += orq_uniform_u32(&seed_eng, 1u, 0xffffffffu);
+        if (g < 3)
+            s->seed[g] = sum;
+    }
+    const uint32_t sz = (m + 7) / 8;
+    uint8_t *mb = (uint8_t *)calloc((size_t)n_masks * sz, 1);
+    uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * n_masks * 3);
+    s->masks = (uint32_t *)malloc(sizeof(uint32_t) * n_masks);
+    s->index = (uint8_t *)malloc((size_t)n_masks * 3);
+    s->split = (uint32_t *)calloc((size_t)n * m * times, sizeof(uint32_t));
+    s->search = (uint32_t *)calloc((size_t)n * m, sizeof(uint32_t));
+    if (!mb || !idx || !s->masks || !s->index || !s->split || !s->search) {
+        free(mb);
+        free(idx);
+        orq_sf_destroy(s);
+        return ORQ_ERR_NOMEM;
+    }
+    orq_generate_hashmask_idx(&mask_eng, n_masks, m, 3, 0, mb, idx); /* v42:268-269 */
+    for (uint32_t i = 0; i < n_masks; ++i) {
+        uint32_t v = 0;
+        for (uint32_t b = 0; b < sz; ++b)
+            v |= (uint32_t)mb[(size_t)i * sz + b] << (8 * b);
+        s->masks[i] = v;
+        for (uint32_t j = 0; j < 3; ++j)
+            s->index[(size_t)i * 3 + j] = (uint8_t)idx[(size_t)i * 3 + j];
+    }
+    free(mb);
+    free(idx);
+    s->n = n;
+    s->m = m;
+    s->times = times;
+    s->n_masks = n_masks;
+    s->key_bytes = key_bytes;
+    return ORQ_OK;
+}
+
+void orq_sf_destroy(orq_sf *s)
+{
+    free(s->masks);
+    free(s->index);
+    free(s->split);
+    free(s->search);
+    memset(s, 0, sizeof(*s));
+}
+
+/* hash_mul_add of seed group g: k * s_g in Hashed_T (32 bits for 32-bit keys) */
+static inline uint64_t sf_hash(const orq_sf *s, uint64_t k, uint32_t g)
+{
+    return s->key_bytes == 4 ? (uint32_t)((uint32_t)k * (uint32_t)s->seed[g]) : k * s->seed[g];
+}
+
+void orq_sf_insert(orq_sf *s, const void *keys, size_t n)
+{
+    for (size_t i = 0; i < n; ++i) { /* v42:330-355 */
+        const uint64_t k = s->key_bytes == 4 ? ((const uint32_t *)keys)[i] : ((const uint64_t *)keys)[i];
+        const uint64_t h = sf_hash(s, k, 0);
+        const uint64_t b = h % s->n;
+        const uint8_t *ix = s->index + (size_t)(h % s->n_masks) * 3;
+        for (uint32_t j = 0; j < 3; ++j) {
+            const uint64_t t = sf_hash(s, k, j) % s->times;
+            s->split[(b * s->m + ix[j]) * s->times + t] += 1;
+        }
+    }
+    /* update() (v42:302-327, update_table v42:72-95): every counter becomes
+     * the largest of its splits */
+    const size_t cells = (size_t)s->n * s->m;
+    for (size_t c = 0; c < cells; ++c) {
+        uint32_t v = 0;
+        for (uint32_t t = 0; t < s->times; ++t)
+            v = s->split[c * s->times + t] > v ? s->split[c * s->times + t] : v;
+        s->search[c] = v;
+    }
+}
+
+void orq_sf_query(const orq_sf *s, const void *keys, size_t n, uint32_t *out)
+{
+    for (size_t i = 0; i < n; ++i) { /* v42 search_kernel, calculation_s2_search v42:98-118 */
+        const uint64_t k = s->key_bytes == 4 ? ((const uint32_t *)keys)[i] : ((const uint64_t *)keys)[i];
+        const uint64_t h = sf_hash(s, k, 0);
+        const uint32_t *row = s->search + (h % s->n) * s->m;
+        uint32_t v = 0xffffffffu;
+        for (uint32_t M = s->masks[h % s->n_masks]; M; M &= M - 1u) {
+            const uint32_t c = row[__builtin_ctz(M)];
+            v = c < v ? c : v;
+        }
+        out[i] = v;
+    }
 }
 
 size_t orq_exact_counts(const void *keys, size_t n, uint32_t key_bytes, void *uniq,

@@ -115,6 +115,38 @@ void orq_sl_destroy(orq_sl *s);
 void orq_sl_insert(orq_sl *s, const void *keys, size_t n);
 void orq_sl_query(const orq_sl *s, const void *keys, size_t n, uint32_t *out);
 
+/* ---- SF sketch with split counters (device_sketch_v42.h:207-480) -------- */
+/* n buckets of m counters (m in {8, 16, 32}), each counter split into `times`
+ * sub-counters.  Seeds: 1 + 3 groups of seed_sz draws (v42 ds.resize(1 +
+ * HASH_MASK_ONES, ss)); group g acts through its sum s_g.  Masks:
+ * generate_hashmask(n_masks, m, 3, 0) with the index of each mask's bits in
+ * draw order.  Insert of k (v42 insert, host loop): h = k*s_0; b = h % n;
+ * mu = h % n_masks; for j = 0..2: split[b][idx[mu][j]][(k*s_j) % times] += 1.
+ * After each insert call (v42 update, update_table): search[b][c] =
+ * max_t split[b][c][t].  Query (v42 search_kernel): min over the mask's bits c
+ * of search[b][c]. */
+typedef struct {
+    uint64_t  n;
+    uint32_t  m;
+    uint32_t  times;
+    uint32_t  n_masks;
+    uint32_t  key_bytes;
+    uint64_t  seed[3];   /* group sums (64-bit, as hash_mul_add sums for 64-bit keys) */
+    uint32_t *masks;     /* bits 0..m-1 */
+    uint8_t  *index;     /* n_masks * 3: each mask's bits in draw order */
+    uint32_t *split;     /* n * m * times */
+    uint32_t *search;    /* n * m */
+} orq_sf;
+int  orq_sf_create(orq_sf *s, uint64_t n, uint32_t m, uint32_t times, uint32_t n_masks, uint32_t seed_sz,
+                   uint32_t key_bytes, uint64_t rng_seed);
+void orq_sf_destroy(orq_sf *s);
+void orq_sf_insert(orq_sf *s, const void *keys, size_t n);
+void orq_sf_query(const orq_sf *s, const void *keys, size_t n, uint32_t *out);
+/* generate_hashmask with the index output (lib/hashmask.cu:32-82): each
+ * mask's ones in the order they were drawn */
+void orq_generate_hashmask_idx(orq_minstd *eng, uint32_t n, uint32_t m_bits, uint32_t ones, uint32_t padding,
+                               uint8_t *out, uint32_t *index);
+
 /* ---- exact counting (sort + reduce-by-key) ------------------------------ */
 /* returns number of distinct keys; uniq/counts sized n by the caller */
 size_t orq_exact_counts(const void *keys, size_t n, uint32_t key_bytes,

@@ -158,6 +158,14 @@ _SIGS = {
     "qs_get_insert_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "qs_autotune_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                      C.POINTER(qs_tune_result)]),
+    # SF sketch (include/qsketch_sf.h)
+    "qs_sf_create": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                               C.POINTER(C.c_void_p)]),
+    "qs_sf_destroy": (None, [C.c_void_p]),
+    "qs_sf_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "qs_sf_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "qs_sf_clear": (C.c_int, [C.c_void_p]),
+    "qs_sf_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     # calibration kernels (include/qsketch_calib.h)
     "qs_calib_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "qs_calib_gather": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p,
@@ -680,6 +688,67 @@ class Sketch_GPU_Sub_Warp:
     def destroy(self) -> None:
         if getattr(self, "_h", None):
             _lib().qs_sl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Sketch_GPU_SF_Host_Sub_Warp:
+    """SF sketch with split counters (include/qsketch_sf.h), mirroring the
+    reference's Sketch_GPU_SF_Host_Sub_Warp(n, m, times, seed_sz,
+    n_hash_mask_table) (device_sketch_v42.h:245-276): n buckets of m counters,
+    each split `times` ways; a counter reads as its largest split, an estimate
+    is the smallest of the mask's counters."""
+
+    def __init__(self, n: int, m: int = 8, times: int = 3, seed_sz: int = SEED_SZ,
+                 n_hash_mask_table: int = HASH_MASK_TABLE_SZ, *, key_bytes: int = 4, rng_seed: int = 1):
+        h = C.c_void_p()
+        check(_lib().qs_sf_create(int(n), int(m), int(times), int(seed_sz), int(n_hash_mask_table),
+                                  int(key_bytes), int(rng_seed), C.byref(h)), "qs_sf_create")
+        self._h = h
+        self.n, self.m, self.times = int(n), int(m), int(times)
+        self.n_masks, self.key_bytes = int(n_hash_mask_table), int(key_bytes)
+
+    def insert(self, keys, keys_sz: int | None = None) -> int:
+        kp, n = _keys_arg(keys, keys_sz, self.key_bytes, "qs_sf_insert")
+        check(_lib().qs_sf_insert(self._h, kp, n), "qs_sf_insert")
+        return 0
+
+    def search(self, keys, keys_sz: int | None = None, counts=None):
+        import torch
+        kp, n = _keys_arg(keys, keys_sz, self.key_bytes, "qs_sf_query")
+        if counts is None:
+            counts = torch.empty(n, dtype=torch.int32, device=keys.device)
+        check(_lib().qs_sf_query(self._h, kp, n, _counts_arg(counts, n, "qs_sf_query")), "qs_sf_query")
+        return counts
+
+    query = search
+
+    def clear(self) -> None:
+        check(_lib().qs_sf_clear(self._h), "qs_sf_clear")
+
+    def export_state(self):
+        import numpy as np
+        split = np.zeros(self.n * self.m * self.times, np.uint32)
+        search = np.zeros(self.n * self.m, np.uint32)
+        masks = np.zeros(self.n_masks, np.uint32)
+        index = np.zeros(self.n_masks * 3, np.uint8)
+        seeds = np.zeros(3, np.uint64)
+        check(_lib().qs_sf_export(self._h, split.ctypes.data, search.ctypes.data, masks.ctypes.data,
+                                  index.ctypes.data, seeds.ctypes.data), "qs_sf_export")
+        return {"split": split, "search": search, "masks": masks, "index": index,
+                "seeds": [int(x) for x in seeds]}
+
+    def memory_bytes(self) -> int:
+        return self.n * self.m * (self.times + 1) * 4
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().qs_sf_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqsketch_amd.so")
 SOURCES = ["qs_sketch.hip", "qs_insert.hip", "qs_sort.hip", "qs_gen.hip", "qs_learned.hip", "qs_single.hip",
-           "qs_calib.hip", "qs_hot.hip"]
+           "qs_calib.hip", "qs_hot.hip", "qs_sf.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QS_OFFLOAD_ARCH", "gfx950")

@@ -265,3 +265,13 @@ def test_accuracy_metrics_on_hand_computed_cases():
     # a key that was never inserted counts in AAE, not in ARE
     m = acc.metrics(torch.tensor([4, 2], dtype=torch.int32), torch.tensor([2, 0]))
     assert m["are"] == pytest.approx(1.0) and m["aae"] == pytest.approx(2.0) and m["absent_keys"] == 1
+
+
+def test_sf_create_rejects_bad_parameters():
+    import qsketch_amd as q
+    L = q.lib()
+    h = C.c_void_p()
+    for args in ((1021, 7, 3, 1, 1021, 4), (1021, 8, 0, 1, 1021, 4), (1021, 8, 3, 0, 1021, 4),
+                 (1021, 8, 3, 1, 0, 4), (1021, 8, 3, 1, 5000, 4), (1021, 8, 3, 1, 1021, 2),
+                 (1021, 8, 3, 2, 1021, 8), (0, 8, 3, 1, 1021, 4)):
+        assert L.qs_sf_create(*args, 1, C.byref(h)) == q.QS_ERR_PARAM, args

@@ -416,3 +416,87 @@ def test_single_level_seed_and_masks_follow_the_two_level_engines():
     m = po.Minstd(seed_bytes=po.seed_buffer(9, 1))
     ref = m.hashmask(1021, 8, 3, 0)
     assert np.array_equal(o1.masks(), ref.astype(np.uint32).reshape(-1))
+
+
+# ---------------------------------------------------------------------------
+# SF sketch (device_sketch_v42.h:207-480)
+# ---------------------------------------------------------------------------
+def _naive_sf(keys, n, m, times, seeds, masks, index, key_bytes=4):
+    """The SF sketch written from v42's insert loop and update(), with Python
+    integers and dicts (independent of the C restatement)."""
+    mod = (1 << 32) if key_bytes == 4 else (1 << 64)
+    split = {}
+    for k in (int(x) for x in keys):
+        h = (k * (seeds[0] % mod)) % mod
+        b, mu = h % n, h % len(masks)
+        for j in range(3):
+            t = ((k * (seeds[j] % mod)) % mod) % times
+            c = int(index[mu * 3 + j])
+            split[(b, c, t)] = split.get((b, c, t), 0) + 1
+    search = {}
+    for (b, c, t), v in split.items():
+        search[(b, c)] = max(search.get((b, c), 0), v)
+
+    def query(k):
+        h = (int(k) * (seeds[0] % mod)) % mod
+        b, mu = h % n, h % len(masks)
+        return min(search.get((b, c), 0) for c in range(m) if int(masks[mu]) >> c & 1)
+    return split, search, query
+
+
+@pytest.mark.parametrize("m,times,key_bytes", [(8, 3, 4), (16, 8, 4), (32, 2, 8), (8, 5, 8)])
+def test_sf_matches_naive_model(m, times, key_bytes):
+    rs = np.random.default_rng(m * times + key_bytes)
+    dt = np.uint32 if key_bytes == 4 else np.uint64
+    keys = rs.integers(0, 1 << 20, 3000).astype(dt) * dt(2654435761 if key_bytes == 4 else 11400714819323198485)
+    keys = np.concatenate([keys, np.repeat(keys[:5], 40)])  # a few frequent keys
+    rs.shuffle(keys)
+    n = 61
+    o = po.OracleSF(n, m, times, 1021, key_bytes=key_bytes, rng_seed=4)
+    o.insert(keys[:2000])
+    o.insert(keys[2000:])  # two calls: the update runs after each
+    split, search, query = _naive_sf(keys, n, m, times, o.seeds(), o.masks(), o.index(), key_bytes)
+    want = np.zeros(n * m * times, np.uint32)
+    for (b, c, t), v in split.items():
+        want[(b * m + c) * times + t] = v
+    assert np.array_equal(o.split(), want)
+    ws = np.zeros(n * m, np.uint32)
+    for (b, c), v in search.items():
+        ws[b * m + c] = v
+    assert np.array_equal(o.search(), ws)
+    q = np.concatenate([keys[:200], keys[:20] ^ dt(77)])
+    assert [int(x) for x in o.query(q)] == [query(k) for k in q]
+    # invariants: 3 increments per key; the estimate never falls below the count
+    assert int(o.split().sum()) == 3 * keys.size
+    u, c = np.unique(keys, return_counts=True)
+    assert np.all(o.query(u) >= c)
+
+
+def test_sf_mask_index_is_the_draw_order():
+    """The index table lists each mask's bits in the order generate_hashmask
+    drew them: replaying the draws with the engine restatement gives it."""
+    o = po.OracleSF(101, 16, 3, 1021, rng_seed=9)
+    eng = po.Minstd(seed_bytes=po.seed_buffer(9, 1))
+    idx, masks = o.index(), o.masks()
+    for i in range(1021):
+        got = []
+        while len(got) < 3:
+            d = eng.uniform(0, 15)
+            if d not in got:
+                got.append(d)
+        assert got == [int(x) for x in idx[i * 3:i * 3 + 3]], i
+        assert int(masks[i]) == sum(1 << d for d in got)
+
+
+def test_sf_with_one_split_is_the_single_level_sketch():
+    """times = 1: every counter has one split, so the SF sketch is the
+    single-level sketch with the same seed draws and masks."""
+    rs = np.random.default_rng(5)
+    keys = rs.integers(0, 1 << 16, 40000).astype(np.uint32)
+    for m in (8, 16, 32):
+        sf = po.OracleSF(4093, m, 1, 1021, seed_sz=2, rng_seed=6)
+        sl = po.OracleSingleLevel(4093, m, 1021, seed_sz=2, rng_seed=6)
+        sf.insert(keys)
+        sl.insert(keys)
+        assert np.array_equal(sf.search(), sl.table())
+        assert np.array_equal(sf.query(keys[:500]), sl.query(keys[:500]))
