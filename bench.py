@@ -558,7 +558,6 @@ def bench_sf(a):
         ti.append(timed(lambda: sk.insert(keys)))
         tq.append(timed(lambda: sk.search(uniq, uniq.numel(), counts)))
     t_i, t_q, t_c = (float(np.median(x)) for x in (ti, tq, tc))
-    upd_bytes = n * 8 * 4 * 4  # the update streams 3 splits and writes the largest, per counter
     # CPU: the oracle, one thread, on a bounded prefix with a table sized for it
     hk = keys[:1 << 22].cpu().numpy().view(np.uint32).copy()
     hu = np.unique(hk)
@@ -580,7 +579,6 @@ def bench_sf(a):
                    "keys": N, "distinct": int(uniq.numel()), "memory_bytes": sk.memory_bytes()},
         "insert_gkeys_s": round(N / t_i / 1e9, 3), "insert_ms": round(t_i * 1e3, 3),
         "query_gkeys_s": round(uniq.numel() / t_q / 1e9, 3), "query_ms": round(t_q * 1e3, 3),
-        "update_bytes_per_insert_call": upd_bytes,
         "cpu_baseline": {"value": round((hk.size + hu.size) / (t2 - t0) / 1e9, 5), "unit": "Gkeys/s", "cores": 1,
                          "kind": "port",
                          "sample": f"first 2^22 keys of the same stream into a table sized for them, then their "

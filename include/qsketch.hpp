@@ -4,6 +4,8 @@
 //
 //   qsketch::Sketch<Key_T, Count_T>            include/sketch/sketch.h:5-28
 //   qsketch::Sketch_GPU_Sub_Warp_Level<...>    include/sketch/device_sketch/device_sketch_v3.h:524-634
+//   qsketch::Sketch_GPU_Sub_Warp<...>          include/sketch/device_sketch/device_sketch_v2.h:289-410
+//   qsketch::Sketch_GPU_SF_Host_Sub_Warp<...>  include/sketch/device_sketch/device_sketch_v42.h:207-480
 //   qsketch::default_values                    include/default.h, lib/default.cu:29-63
 //   Sketch_GPU::number_of_buckets              include/sketch/device_sketch/device_sketch.h:371-376
 //
@@ -22,6 +24,8 @@
 #include <type_traits>
 
 #include "qsketch.h"
+#include "qsketch_sf.h"
+#include "qsketch_single.h"
 
 namespace qsketch {
 
@@ -124,6 +128,74 @@ struct Sketch_GPU_Sub_Warp_Level : public Sketch<Key_T, Count_T> {
     {
         return ((double)params.n_low * 32.0 + (double)params.n_high * 128.0) / (1 << 20);
     }
+};
+
+// The single-level sub-warp sketch: n buckets of m counters, count-min.
+template <typename Key_T, typename Count_T, typename Hashed_T = Key_T, typename Seed_T = unsigned,
+          typename Hash_Function = void>
+struct Sketch_GPU_Sub_Warp : public Sketch<Key_T, Count_T> {
+    static_assert(sizeof(Key_T) == 4 || sizeof(Key_T) == 8, "keys are 32- or 64-bit");
+    static_assert(sizeof(Count_T) == 4 && std::is_unsigned<Count_T>::value, "counts are uint32");
+
+    qs_sl *h = nullptr;
+
+    // v2:307-309
+    Sketch_GPU_Sub_Warp(std::size_t _n, std::size_t _m, std::size_t _ss = default_values::seed_sz,
+                        std::size_t _n_hash_mask_table = default_values::HASH_MASK_TABLE_SZ,
+                        std::uint64_t rng_seed = 1)
+    {
+        check(qs_sl_create(_n, (std::uint32_t)_m, (std::uint32_t)_ss, (std::uint32_t)_n_hash_mask_table,
+                           sizeof(Key_T), rng_seed, &h),
+              "qs_sl_create");
+    }
+    Sketch_GPU_Sub_Warp(const Sketch_GPU_Sub_Warp &) = delete;
+    Sketch_GPU_Sub_Warp &operator=(const Sketch_GPU_Sub_Warp &) = delete;
+    ~Sketch_GPU_Sub_Warp() override { qs_sl_destroy(h); }
+
+    int insert(Key_T *keys, std::size_t keys_sz) override
+    {
+        return check(qs_sl_insert(h, keys, keys_sz), "qs_sl_insert");
+    }
+    int search(Key_T *keys, std::size_t keys_sz, Count_T *count) override
+    {
+        return check(qs_sl_query(h, keys, keys_sz, reinterpret_cast<std::uint32_t *>(count)), "qs_sl_query");
+    }
+    void clear() override { check(qs_sl_clear(h), "qs_sl_clear"); }
+};
+
+// The SF sketch: n buckets of m counters, each split `times` ways; insert and
+// the update of the counters run on the GPU.
+template <typename Key_T, typename Count_T, typename Hashed_T = Key_T, typename Seed_T = unsigned,
+          typename Hash_Function = void>
+struct Sketch_GPU_SF_Host_Sub_Warp : public Sketch<Key_T, Count_T> {
+    static_assert(sizeof(Key_T) == 4 || sizeof(Key_T) == 8, "keys are 32- or 64-bit");
+    static_assert(sizeof(Count_T) == 4 && std::is_unsigned<Count_T>::value, "counts are uint32");
+
+    qs_sf *h = nullptr;
+
+    // v42:242-243
+    Sketch_GPU_SF_Host_Sub_Warp(std::size_t _n, std::size_t _m, unsigned _times,
+                                std::size_t _ss = default_values::seed_sz,
+                                std::size_t _n_hash_mask_table = default_values::HASH_MASK_TABLE_SZ,
+                                std::uint64_t rng_seed = 1)
+    {
+        check(qs_sf_create(_n, (std::uint32_t)_m, _times, (std::uint32_t)_ss, (std::uint32_t)_n_hash_mask_table,
+                           sizeof(Key_T), rng_seed, &h),
+              "qs_sf_create");
+    }
+    Sketch_GPU_SF_Host_Sub_Warp(const Sketch_GPU_SF_Host_Sub_Warp &) = delete;
+    Sketch_GPU_SF_Host_Sub_Warp &operator=(const Sketch_GPU_SF_Host_Sub_Warp &) = delete;
+    ~Sketch_GPU_SF_Host_Sub_Warp() override { qs_sf_destroy(h); }
+
+    int insert(Key_T *keys, std::size_t keys_sz) override
+    {
+        return check(qs_sf_insert(h, keys, keys_sz), "qs_sf_insert");
+    }
+    int search(Key_T *keys, std::size_t keys_sz, Count_T *count) override
+    {
+        return check(qs_sf_query(h, keys, keys_sz, reinterpret_cast<std::uint32_t *>(count)), "qs_sf_query");
+    }
+    void clear() override { check(qs_sf_clear(h), "qs_sf_clear"); }
 };
 
 } // namespace qsketch

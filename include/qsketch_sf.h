@@ -35,7 +35,12 @@ int qs_sf_create(uint64_t n, uint32_t m, uint32_t times, uint32_t seed_sz, uint3
                  uint64_t rng_seed, qs_sf **out);
 /* ~Sketch_GPU_SF_Host_Sub_Warp (v42:468-478) */
 void qs_sf_destroy(qs_sf *s);
-/* insert (v42:329-384, including the update of v42:358) */
+/* insert (v42:329-384, including the update of v42:358).  Calls of 2^16 keys
+ * or more take scratch of 2 * key_bytes bytes per key (at most 2^28 4-byte or
+ * 2^27 8-byte keys; longer calls run in chunks), kept until destroy.
+ * QS_ERR_INTERNAL: a device-side consistency check failed (the state is
+ * invalid until qs_sf_clear).  A split that passes 2^32 - 1 within one call
+ * wraps, and its counter then need not be the largest split. */
 int qs_sf_insert(qs_sf *s, const void *d_keys, size_t n);
 /* search (v42:385-459) */
 int qs_sf_query(qs_sf *s, const void *d_keys, size_t n, uint32_t *d_counts);

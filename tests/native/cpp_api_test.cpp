@@ -2,7 +2,8 @@
 // Test harness drives Sketch<Key_T, Count_T> (include/test/test.h:137-184):
 // construct a Sketch_GPU_Sub_Warp_Level sized by number_of_buckets, insert
 // device key batches, search, clear -- and checks every estimate against the
-// CPU oracle.  Exit status 0 = all estimates bit-exact.
+// CPU oracle; then the same for Sketch_GPU_Sub_Warp and
+// Sketch_GPU_SF_Host_Sub_Warp.  Exit status 0 = all estimates bit-exact.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -66,6 +67,36 @@ int main()
         std::printf("pool exhaustion reported: %s\n", e.what());
     }
     orq_destroy(&o);
+
+    // the single-level and the SF sketch through the same Sketch& interface
+    {
+        Sketch_GPU_Sub_Warp<unsigned, unsigned> sl(40009, 16, 1, 1021, /*rng_seed=*/5);
+        Sketch_GPU_SF_Host_Sub_Warp<unsigned, unsigned> sf(40009, 8, 3, 1, 1021, /*rng_seed=*/5);
+        Sketch<unsigned, unsigned> *both[2] = {&sl, &sf};
+        orq_sl osl;
+        orq_sf osf;
+        if (orq_sl_create(&osl, 40009, 16, 1021, 1, 4, 5) || orq_sf_create(&osf, 40009, 8, 3, 1021, 1, 4, 5))
+            return 5;
+        orq_sl_insert(&osl, keys.data(), N);
+        orq_sf_insert(&osf, keys.data(), N);
+        for (int v = 0; v < 2; ++v) {
+            both[v]->insert(d_keys, N);
+            both[v]->search(d_keys, N, d_counts);
+            hipMemcpy(est.data(), d_counts, N * 4, hipMemcpyDeviceToHost);
+            if (v == 0)
+                orq_sl_query(&osl, keys.data(), N, ref.data());
+            else
+                orq_sf_query(&osf, keys.data(), N, ref.data());
+            size_t diff = 0;
+            for (size_t i = 0; i < N; ++i)
+                diff += est[i] != ref[i];
+            std::printf("%s: estimates differing from the oracle: %zu of %zu\n",
+                        v == 0 ? "Sketch_GPU_Sub_Warp" : "Sketch_GPU_SF_Host_Sub_Warp", diff, N);
+            bad += diff;
+        }
+        orq_sl_destroy(&osl);
+        orq_sf_destroy(&osf);
+    }
     hipFree(d_keys);
     hipFree(d_counts);
     std::printf("%s\n", bad == 0 && threw ? "OK" : "FAIL");

@@ -94,3 +94,61 @@ def test_sf_large_skewed_stream(gpu):
     assert np.array_equal(est, o.query(uq))
     u, c = np.unique(keys, return_counts=True)
     assert np.all(est >= c)
+
+
+def test_sf_short_and_threshold_calls(gpu):
+    """Calls on both sides of the length where the insert starts grouping the
+    keys by bucket first (2^16), and a 3-key call, into one sketch: the
+    unsorted and the grouped paths leave the oracle's state."""
+    import qsketch_amd as q
+    _case(test="sf_paths")
+    keys = q.Zipf(1.2, 1 << 16, device=False).keys_u32_host(9, 3 + (1 << 16) - 1 + (1 << 16) + 70001)
+    g = q.Sketch_GPU_SF_Host_Sub_Warp(40009, 8, 4, 1, 1021, rng_seed=5)
+    o = po.OracleSF(40009, 8, 4, 1021, rng_seed=5)
+    a = 0
+    for ln in (3, (1 << 16) - 1, 1 << 16, 70001):
+        part = keys[a:a + ln]
+        a += ln
+        g.insert(to_dev(part))
+        o.insert(part)
+        _same(g, o, f"call of {ln} keys")
+    uq = np.unique(keys)
+    assert np.array_equal(to_host_u32(g.search(to_dev(uq))), o.query(uq))
+
+
+@pytest.mark.parametrize("n", [7, 300, 70001, (1 << 24) + 43])
+def test_sf_grouping_for_every_bucket_width(gpu, n):
+    """Tables whose bucket index has 3, 9, 17 and 25 bits: one or two
+    grouping passes, over all of the bits or the top 16."""
+    import qsketch_amd as q
+    _case(test="sf_width", n=n)
+    keys = q.uniform_u32_host(13, 1 << 18)
+    keys[::3] = keys[7]  # a hot key
+    g = q.Sketch_GPU_SF_Host_Sub_Warp(n, 16, 3, 1, 1021, rng_seed=8)
+    o = po.OracleSF(n, 16, 3, 1021, rng_seed=8)
+    g.insert(to_dev(keys))
+    o.insert(keys)
+    _same(g, o, f"n {n}")
+    uq = np.unique(keys)
+    assert np.array_equal(to_host_u32(g.search(to_dev(uq))), o.query(uq))
+
+
+@pytest.mark.slow
+def test_sf_call_longer_than_one_chunk(gpu):
+    """2^28 + 54321 keys in one call: two radix chunks (2^28 keys each at
+    most), into a small table, against the oracle."""
+    import torch
+    import qsketch_amd as q
+    _case(test="sf_chunks", n=(1 << 28) + 54321)
+    N = (1 << 28) + 54321
+    d = q.Zipf(1.1, 1 << 24).keys_u32(3, N)
+    g = q.Sketch_GPU_SF_Host_Sub_Warp(1 << 20, 8, 3, 1, 1021, rng_seed=3)
+    g.insert(d)
+    keys = d.cpu().numpy().view(np.uint32)
+    del d
+    torch.cuda.empty_cache()
+    o = po.OracleSF(1 << 20, 8, 3, 1021, rng_seed=3)
+    o.insert(keys)
+    _same(g, o, "two chunks")
+    uq = np.unique(keys)
+    assert np.array_equal(to_host_u32(g.search(to_dev(uq))), o.query(uq))
