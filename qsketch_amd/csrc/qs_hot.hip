@@ -26,9 +26,14 @@
 // promoting inserts, are unchanged).  A call whose sample shows no skew
 // (candidates covering less than HOT_MIN_COVER of it) runs the ordinary
 // pipeline unchanged.
+//
+// qs_hot_split_keys serves a sink whose inserts all commute (the SF sketch,
+// qs_sf.hip): no prefix and no link condition; every candidate enters the
+// table, and the caller gets the cold keys and the per-slot counts.
 #include "qs_internal.h"
 
 #include <cstring>
+#include <functional>
 
 #define CK(x)                                                                                    \
     do {                                                                                         \
@@ -239,6 +244,35 @@ __global__ void k_hot_build(const uint32_t *__restrict__ cand, qs_sdesc d0, cons
     tkey[slot] = k;
     tinfo[slot] = w;
     tmask[slot] = M;
+    atomicAdd(ctl + HC_NHOT, 1u);
+}
+
+// every candidate enters the hot table (a sink whose inserts all commute)
+__global__ void k_hot_build_any(const uint32_t *__restrict__ cand, uint32_t *__restrict__ tkey, uint32_t *tfill,
+                                uint32_t *ctl)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= min(ctl[HC_NCAND], HOT_MAX))
+        return;
+    const uint32_t k = cand[c];
+    if (k == HOT_EMPTY)
+        return;
+    uint32_t b1, b2;
+    hot_buckets(k, b1, b2);
+    if (__hip_atomic_load(tfill + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+        __hip_atomic_load(tfill + b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        const uint32_t t = b1;
+        b1 = b2;
+        b2 = t;
+    }
+    uint32_t pos = atomicAdd(tfill + b1, 1u), hb = b1;
+    if (pos >= 4u) {
+        pos = atomicAdd(tfill + b2, 1u);
+        hb = b2;
+    }
+    if (pos >= 4u)
+        return; // both full: the key stays cold
+    tkey[hb * 4u + pos] = k;
     atomicAdd(ctl + HC_NHOT, 1u);
 }
 
@@ -464,9 +498,9 @@ __global__ __launch_bounds__(256) void k_hot_flush(const uint32_t *__restrict__ 
 // ===========================================================================
 // host
 // ===========================================================================
-void qs_hot_free(qs_sketch *s)
+void qs_hot_state_free(void *state)
 {
-    qs_hot *h = (qs_hot *)s->hot;
+    qs_hot *h = (qs_hot *)state;
     if (!h)
         return;
     for (int k = 0; k < 2; ++k) {
@@ -488,19 +522,25 @@ void qs_hot_free(qs_sketch *s)
     (void)hipFree(h->cold);
     if (h->h_ctl)
         (void)hipHostFree(h->h_ctl);
-
     free(h);
+}
+
+void qs_hot_free(qs_sketch *s)
+{
+    qs_hot_state_free(s->hot);
     s->hot = nullptr;
 }
 
-static int hot_alloc(qs_sketch *s, uint64_t rest)
+// the filter's scratch in *state (created on first use), sized for `rest`
+// keys after the part inserted the ordinary way
+static int hot_alloc(void **state, uint64_t rest)
 {
-    qs_hot *h = (qs_hot *)s->hot;
+    qs_hot *h = (qs_hot *)*state;
     if (!h) {
         h = (qs_hot *)calloc(1, sizeof(qs_hot));
         if (!h)
             return QS_ERR_NOMEM;
-        s->hot = h;
+        *state = h;
         int dev = 0, cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
@@ -553,27 +593,14 @@ static int hot_alloc(qs_sketch *s, uint64_t rest)
     return QS_OK;
 }
 
-// Blocking insert of n 32-bit keys into a plain sketch through the hot-key
-// filter.  *used = 0 (and nothing inserted) when the call is too short or its
-// sample shows too little skew: the caller then runs the ordinary insert.
-int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stream, int *used)
+// Candidates from the first HOT_SAMPLE keys (step 1): *cand = the candidate
+// keys, most frequent first (at most HOT_MAX are used); *skewed = whether they
+// cover at least HOT_MIN_COVER of the sample.  `launched` (may be empty) runs
+// once the kernels are queued, before the wait for the cover.
+static int hot_candidates(qs_hot *h, const uint32_t *keys, hipStream_t stream, const std::function<void()> &launched,
+                          const uint32_t **cand, bool *skewed)
 {
-    *used = 0;
-    if (s->nsk != 1 || s->key_bytes != 4 || n < HOT_MIN_KEYS || n - HOT_PREFIX > 0xffffffffull)
-        return QS_OK;
-    const uint32_t *keys = (const uint32_t *)d_keys;
-    const uint64_t rest = n - HOT_PREFIX;
-    int rc = hot_alloc(s, rest);
-    if (rc)
-        return rc;
-    qs_hot *h = (qs_hot *)s->hot;
-    const qs_sdesc d0 = s->h_desc[0];
-    qs_evpair e0, e1, e2;
-    const bool timed = s->timing != 0;
-
-    // 1. candidates from the head of the call
-    if (timed)
-        qs_tim_begin(s, e0, QS_TK_HOT, 0, stream);
+    *skewed = false;
     hipLaunchKernelGGL(k_hot_sample, dim3(HOT_SAMPLE / 256), dim3(256), 0, stream, keys, HOT_SAMPLE, h->sk[0],
                        h->sv[0], h->ctl);
     const int r0 = qs_sort_pairs(h->sk, h->sv, h->hist, HOT_SAMPLE, 32, stream);
@@ -592,15 +619,47 @@ int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stre
         return QS_ERR_HIP;
     hipLaunchKernelGGL(k_hot_cover, dim3(1), dim3(1024), 0, stream, k2[r2], h->ctl);
     CK(hipGetLastError());
-    if (timed)
-        qs_tim_end(s, e0, stream);
-    rc = hot_read(h, h->ctl, HC_WORDS, stream);
+    if (launched)
+        launched();
+    const int rc = hot_read(h, h->ctl, HC_WORDS, stream);
     if (rc)
         return rc;
     const uint64_t cover = h->h_ctl[HC_COVER];
-    if (h->h_ctl[HC_NCAND] == 0 || cover * HOT_MIN_COVER_DEN < (uint64_t)HOT_SAMPLE * HOT_MIN_COVER_NUM)
+    *skewed = h->h_ctl[HC_NCAND] != 0 && cover * HOT_MIN_COVER_DEN >= (uint64_t)HOT_SAMPLE * HOT_MIN_COVER_NUM;
+    *cand = v2[r2];
+    return QS_OK;
+}
+
+// Blocking insert of n 32-bit keys into a plain sketch through the hot-key
+// filter.  *used = 0 (and nothing inserted) when the call is too short or its
+// sample shows too little skew: the caller then runs the ordinary insert.
+int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stream, int *used)
+{
+    *used = 0;
+    if (s->nsk != 1 || s->key_bytes != 4 || n < HOT_MIN_KEYS || n - HOT_PREFIX > 0xffffffffull)
+        return QS_OK;
+    const uint32_t *keys = (const uint32_t *)d_keys;
+    const uint64_t rest = n - HOT_PREFIX;
+    int rc = hot_alloc(&s->hot, rest);
+    if (rc)
+        return rc;
+    qs_hot *h = (qs_hot *)s->hot;
+    const qs_sdesc d0 = s->h_desc[0];
+    qs_evpair e0, e1, e2;
+    const bool timed = s->timing != 0;
+
+    // 1. candidates from the head of the call
+    if (timed)
+        qs_tim_begin(s, e0, QS_TK_HOT, 0, stream);
+    const uint32_t *cand = nullptr;
+    bool skewed = false;
+    const std::function<void()> launched = timed ? std::function<void()>([&]() { qs_tim_end(s, e0, stream); })
+                                                 : std::function<void()>();
+    rc = hot_candidates(h, keys, stream, launched, &cand, &skewed);
+    if (rc)
+        return rc;
+    if (!skewed)
         return QS_OK; // not skewed enough to pay for the filter
-    const uint32_t *cand = v2[r2];
 
     // 2. the prefix the ordinary way.  The insert scratch is sized now for
     // every key after the prefix, the most the cold part can hold: the cold
@@ -663,4 +722,47 @@ int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stre
     }
     const uint64_t offs_c[2] = {0, ncold};
     return qs_insert_set(s, h->cold, offs_c, nullptr, stream, true, false);
+}
+
+// For a sink whose inserts all commute (the SF sketch): the n keys split into
+// hot keys, counted per table slot and persistent block, and cold keys in
+// stream order.  out->split = 0 (nothing done) when n < HOT_MIN_KEYS or the
+// sample shows too little skew.
+int qs_hot_split_keys(void **state, const uint32_t *keys, uint64_t n, hipStream_t stream, qs_hot_split *out)
+{
+    memset(out, 0, sizeof(*out));
+    if (n < HOT_MIN_KEYS || n > 0xffffffffull)
+        return QS_OK;
+    int rc = hot_alloc(state, n);
+    if (rc)
+        return rc;
+    qs_hot *h = (qs_hot *)*state;
+    const uint32_t *cand = nullptr;
+    bool skewed = false;
+    rc = hot_candidates(h, keys, stream, std::function<void()>(), &cand, &skewed);
+    if (rc || !skewed)
+        return rc;
+    const uint32_t nchunks = (uint32_t)((n + HOT_CHUNK - 1) / HOT_CHUNK);
+    hipLaunchKernelGGL(k_hot_clear, dim3(HOT_SLOTS / 256), dim3(256), 0, stream, h->tkey, h->tfill, h->ctl);
+    hipLaunchKernelGGL(k_hot_build_any, dim3(HOT_MAX / 256), dim3(256), 0, stream, cand, h->tkey, h->tfill, h->ctl);
+    hipLaunchKernelGGL(k_hot_count, dim3(h->nblk), dim3(HOT_THREADS), 0, stream, keys, n, nchunks, h->tkey, h->part,
+                       h->cmask, h->ccold);
+    const uint32_t nb = (nchunks + SCAN_SPAN - 1) / SCAN_SPAN;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(1024), 0, stream, h->ccold, nchunks, h->bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, h->bsum, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, stream, h->ccold, nchunks, h->bsum, nb);
+    hipLaunchKernelGGL(k_hot_compact, dim3(h->nblk * 8u), dim3(256), 0, stream, keys, n, nchunks, h->cmask, h->ccold,
+                       h->cold);
+    CK(hipGetLastError());
+    rc = hot_read(h, h->ccold + nchunks, 1, stream);
+    if (rc)
+        return rc;
+    out->split = 1;
+    out->cold = h->cold;
+    out->ncold = h->h_ctl[0];
+    out->part = h->part;
+    out->nblk = h->nblk;
+    out->tkey = h->tkey;
+    out->nslots = HOT_SLOTS;
+    return QS_OK;
 }

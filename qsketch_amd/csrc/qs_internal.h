@@ -321,6 +321,21 @@ struct qs_sketch {
 // sketch; *used = 0 and nothing inserted when the call does not qualify
 int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stream, int *used);
 void qs_hot_free(qs_sketch *s);
+// the same filter for a sink whose inserts all commute: the keys split into
+// hot keys, counted per table slot (tkey[slot]; part[b * nslots + slot] for
+// persistent block b < nblk, summed by the caller) and cold keys (stream
+// order), all in the filter's scratch *state until the next call
+struct qs_hot_split {
+    int split;             // 0: nothing split (short call or too little skew)
+    const uint32_t *cold;
+    uint64_t ncold;
+    const uint32_t *part;
+    uint32_t nblk;
+    const uint32_t *tkey;
+    uint32_t nslots;
+};
+int qs_hot_split_keys(void **state, const uint32_t *keys, uint64_t n, hipStream_t stream, qs_hot_split *out);
+void qs_hot_state_free(void *state);
 
 // Stable LSD radix sort of (key, value) uint32 pairs, 8 bits per pass, over the
 // low key_bits bits.  Input in keys[0]/vals[0]; returns the index (0 or 1) of

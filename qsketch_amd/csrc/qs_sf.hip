@@ -29,6 +29,9 @@
 // split: after the call every counter is what update() leaves.  Inserts
 // shorter than SF_SORT_MIN skip the sort and go to k_sf_runs (runs of equal
 // adjacent keys, atomic adds that return the old value, an atomic max).
+// Long skewed calls of 4-byte keys first pass the hot-key filter of
+// qs_hot.hip (qs_hot_split_keys): the frequent keys are counted in LDS and
+// added once each by k_sf_hot_flush, and only the cold keys are sorted.
 // k_sf_query reads the key's bucket in 16-byte pieces, m / 4 lanes per key,
 // and takes the minimum of the mask's counters.
 #include "qs_internal.h"
@@ -70,6 +73,7 @@ struct qs_sf {
     uint32_t *d_err;    // [0] set by a scatter position outside the chunk, [1] work counter
     uint32_t *d_shared; // two buckets per range of k_sf_rows (SF_NO_BUCKET: none)
     uint64_t scratch_keys;
+    void *hot;          // the hot-key filter's scratch (qs_hot.hip)
 };
 
 struct sf_par {
@@ -661,6 +665,30 @@ __global__ __launch_bounds__(SF_THREADS) void k_sf_rows(const KeyT<KB> *__restri
     }
 }
 
+// The hot keys of a filtered call (qs_hot_split_keys): each slot's count
+// summed over the persistent blocks and added with sf_add (after the cold
+// keys' kernels, so no plain write of k_sf_rows runs beside these atomics).
+// A block takes 64 slots, its four wavefronts a quarter of the rows each.
+__global__ __launch_bounds__(256) void k_sf_hot_flush(const uint32_t *__restrict__ part, uint32_t nblk,
+                                                      const uint32_t *__restrict__ tkey, uint32_t nslots, sf_par p,
+                                                      const uint32_t *__restrict__ index,
+                                                      uint32_t *__restrict__ split, uint32_t *__restrict__ search)
+{
+    __shared__ uint32_t s_sum[4][64];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t sl = blockIdx.x * 64u + lane;
+    uint32_t c = 0;
+    for (uint32_t b = wv; b < nblk; b += 4u)
+        c += part[(size_t)b * nslots + sl];
+    s_sum[wv][lane] = c;
+    __syncthreads();
+    if (wv != 0)
+        return;
+    c = s_sum[0][lane] + s_sum[1][lane] + s_sum[2][lane] + s_sum[3][lane];
+    if (c) // a slot without a key counts nothing
+        sf_add<4>(split, search, p, index, tkey[sl], c);
+}
+
 // ===========================================================================
 // host
 // ===========================================================================
@@ -695,6 +723,7 @@ extern "C" void qs_sf_destroy(qs_sf *s)
     (void)hipFree(s->d_hist);
     (void)hipFree(s->d_shared);
     (void)hipFree(s->d_err);
+    qs_hot_state_free(s->hot);
     free(s->h_masks);
     free(s->h_index);
     free(s);
@@ -852,6 +881,32 @@ static void sf_launch_flush(qs_sf *s, const sf_par &p, const KeyT<KB> *keys, uin
 }
 
 // one chunk: sort by bucket (8-bit digits at most), then flush
+template <int KB> static int sf_insert_chunk(qs_sf *s, const sf_par &p, const KeyT<KB> *keys, uint32_t cn);
+
+// One chunk of 4-byte keys through the hot-key filter: the frequent keys are
+// counted in LDS and added once each (k_sf_hot_flush), only the cold keys
+// are sorted.  *done = 0 when the chunk is short or not skewed.
+static int sf_insert_hot(qs_sf *s, const sf_par &p, const uint32_t *keys, uint32_t cn, int *done)
+{
+    *done = 0;
+    qs_hot_split hs;
+    int rc = qs_hot_split_keys(&s->hot, keys, cn, 0, &hs);
+    if (rc || !hs.split)
+        return rc;
+    *done = 1;
+    if (hs.ncold > cn)
+        return QS_ERR_INTERNAL;
+    if (hs.ncold) {
+        rc = sf_insert_chunk<4>(s, p, hs.cold, (uint32_t)hs.ncold);
+        if (rc)
+            return rc;
+    }
+    hipLaunchKernelGGL(k_sf_hot_flush, dim3(hs.nslots / 64u), dim3(256), 0, 0, hs.part, hs.nblk, hs.tkey, hs.nslots,
+                       p, s->d_index, s->d_split, s->d_search);
+    CK(hipGetLastError());
+    return QS_OK;
+}
+
 template <int KB> static int sf_insert_chunk(qs_sf *s, const sf_par &p, const KeyT<KB> *keys, uint32_t cn)
 {
     const int bits = s->n > 1 ? 64 - __builtin_clzll(s->n - 1) : 0; // bits of the largest bucket
@@ -895,9 +950,14 @@ extern "C" int qs_sf_insert(qs_sf *s, const void *d_keys, size_t n)
     }
     for (uint64_t off = 0; off < n; off += chunk) {
         const uint32_t cn = (uint32_t)(n - off < chunk ? n - off : chunk);
-        const int rc = s->key_bytes == 4
-                           ? sf_insert_chunk<4>(s, p, static_cast<const uint32_t *>(d_keys) + off, cn)
-                           : sf_insert_chunk<8>(s, p, static_cast<const uint64_t *>(d_keys) + off, cn);
+        int rc = QS_OK, done = 0;
+        if (s->key_bytes == 4) {
+            rc = sf_insert_hot(s, p, static_cast<const uint32_t *>(d_keys) + off, cn, &done);
+            if (rc == QS_OK && !done)
+                rc = sf_insert_chunk<4>(s, p, static_cast<const uint32_t *>(d_keys) + off, cn);
+        } else {
+            rc = sf_insert_chunk<8>(s, p, static_cast<const uint64_t *>(d_keys) + off, cn);
+        }
         if (rc != QS_OK)
             return rc;
     }
