@@ -52,7 +52,7 @@
 #define HOT_THREADS 1024
 #define HOT_IPT 16u                        // rows of 64 keys per chunk
 #define HOT_CHUNK (64u * HOT_IPT)          // keys per chunk (one wavefront's unit of work)
-#define HOT_PREFIX (1ull << 25)            // keys inserted the ordinary way first
+#define HOT_PREFIX (1ull << 24)            // keys inserted the ordinary way first
 #define HOT_MIN_KEYS (1ull << 26)          // shorter calls are not filtered
 #define HOT_MIN_COVER_NUM 3u               // filter when the candidates cover at least
 #define HOT_MIN_COVER_DEN 10u              //   3/10 of the sample
@@ -583,7 +583,7 @@ static int hot_alloc(void **state, uint64_t rest)
         h->chunk_cap = chunks;
     }
     if (h->cold_cap < rest) {
-        hipFree(h->cold);
+        (void)hipFree(h->cold);
         h->cold = nullptr;
         h->cold_cap = 0;
         if (hipMalloc(&h->cold, 4ull * rest) != hipSuccess)
