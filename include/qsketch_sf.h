@@ -30,7 +30,10 @@ extern "C" {
 typedef struct qs_sf qs_sf;
 
 /* Sketch_GPU_SF_Host_Sub_Warp(_n, _m, _times, _ss, _n_hash_mask_table)
- * (v42:245-276); rng_seed replaces the reference's /dev/urandom engines. */
+ * (v42:245-276); rng_seed replaces the reference's /dev/urandom engines.
+ * QS_ERR_PARAM unless 1 <= n < 2^32, m in {8, 16, 32}, 1 <= times <= 256,
+ * n * m * times <= 2^36, 1 <= n_masks <= 4096, key_bytes 4 or 8 (8: seed_sz
+ * 1). */
 int qs_sf_create(uint64_t n, uint32_t m, uint32_t times, uint32_t seed_sz, uint32_t n_masks, uint32_t key_bytes,
                  uint64_t rng_seed, qs_sf **out);
 /* ~Sketch_GPU_SF_Host_Sub_Warp (v42:468-478) */

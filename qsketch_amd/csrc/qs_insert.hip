@@ -631,16 +631,22 @@ __device__ static inline bool apply_segment(uint32_t *__restrict__ low, uint32_t
     uint32_t W[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
     uint32_t w = W[7];
     if (w > QS_LINK_START) {
-        uint32_t *row = high + (size_t)(w - QS_LINK_START) * 32u;
+        // the owner alone writes this high row: every touched 16-byte group
+        // is read first (the loads are in flight together), then written
+        uint4 *row4 = reinterpret_cast<uint4 *>(high + (size_t)(w - QS_LINK_START) * 32u);
+        uint4 q[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (a[k])
+                q[k] = row4[k];
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             if (a[k]) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    uint32_t c = (a[k] >> (8 * e)) & 0xffu;
-                    if (c)
-                        row[4 * k + e] += c;
-                }
+                q[k].x += a[k] & 0xffu;
+                q[k].y += (a[k] >> 8) & 0xffu;
+                q[k].z += (a[k] >> 16) & 0xffu;
+                q[k].w += a[k] >> 24;
+                row4[k] = q[k];
             }
         }
         return false;
@@ -852,24 +858,25 @@ __device__ static inline void add_words(uint32_t a[7], const uint32_t b[7])
         a[k] += b[k];
 }
 
-// one owner finishes a run: apply it, or queue it for the replay, or mark it long
+// one owner finishes a run: apply it, or queue it for the replay, or mark it
+// long; returns FR_* bits (a value, not references, so that the flags stay
+// in registers)
+#define FR_RISKY 1u    // to the in-order replay
+#define FR_LONG 2u     // to the tail kernel
+#define FR_UNLINKED 4u // long and unlinked: to the replay first
 template <bool MULTI>
-__device__ static inline void finish_run(bool act, uint32_t sid, uint32_t b, uint64_t len, const uint32_t a[7],
-                                         const qs_sdesc *__restrict__ D, uint32_t *low_pool,
-                                         uint32_t *high_pool, const uint32_t *mask_pool, uint32_t *longbm,
-                                         bool &risky, bool &lng_flag, bool &unl)
+__device__ static inline uint32_t finish_run(bool act, uint32_t sid, uint32_t b, uint64_t len, const uint32_t a[7],
+                                             const qs_sdesc *__restrict__ D, uint32_t *low_pool,
+                                             uint32_t *high_pool, const uint32_t *mask_pool, uint32_t *longbm)
 {
-    risky = lng_flag = unl = false;
     if (!act)
-        return;
+        return 0u;
     seg_tables tb = tables_of<MULTI>(sid, D, low_pool, high_pool, mask_pool);
     if (len >= 256u) {
         set_long<MULTI>(longbm, D, sid, b);
-        lng_flag = true;
-        unl = tb.low[(size_t)b * 8u + 7u] <= QS_LINK_START;
-        return;
+        return FR_LONG | (tb.low[(size_t)b * 8u + 7u] <= QS_LINK_START ? FR_UNLINKED : 0u);
     }
-    risky = apply_segment(tb.low, tb.high, b, a);
+    return apply_segment(tb.low, tb.high, b, a) ? FR_RISKY : 0u;
 }
 
 template <bool MULTI, int W, int IPL>
@@ -958,10 +965,10 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             const bool bnd = c[k] != c[k - 1];
             // an interior run [rs, k) with rs >= 1: its head is in this lane
             const bool inner = bnd && rs > 0 && c[k - 1] != NO_CELL;
-            bool risky = false, lf = false, un = false;
-            finish_run<MULTI>(inner, (uint32_t)(c[k - 1] >> pk.bb), (uint32_t)(c[k - 1] & pk.bmask), 1u, cur, D,
-                              low_pool, high_pool, mask_pool, longbm, risky, lf, un);
-            wave_append(risky, (uint32_t)(p0 + rs), ord, &st->n_ord);
+            const uint32_t fr = finish_run<MULTI>(inner, (uint32_t)(c[k - 1] >> pk.bb),
+                                                  (uint32_t)(c[k - 1] & pk.bmask), 1u, cur, D, low_pool, high_pool,
+                                                  mask_pool, longbm);
+            wave_append((fr & FR_RISKY) != 0u, (uint32_t)(p0 + rs), ord, &st->n_ord);
             if (bnd && rs == 0) {
 #pragma unroll
                 for (int j = 0; j < 7; ++j)
@@ -1018,25 +1025,23 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
                 for (int k = 0; k < 7; ++k)
                     af[k] += carry[k];
             }
-            bool risky, lf, un;
-            finish_run<MULTI>(act, (uint32_t)(c[0] >> pk.bb), (uint32_t)(c[0] & pk.bmask),
-                              act ? p0 + f1 - (head - 1u) : 0u, af, D, low_pool, high_pool, mask_pool, longbm,
-                              risky, lf, un);
+            const uint32_t fr = finish_run<MULTI>(act, (uint32_t)(c[0] >> pk.bb), (uint32_t)(c[0] & pk.bmask),
+                                                  act ? p0 + f1 - (head - 1u) : 0u, af, D, low_pool, high_pool,
+                                                  mask_pool, longbm);
             const uint32_t hpos = (uint32_t)(head - 1u);
-            wave_append(risky || un, hpos, ord, &st->n_ord);
-            wave_append(lf, hpos, lng, &st->n_long);
+            wave_append((fr & (FR_RISKY | FR_UNLINKED)) != 0u, hpos, ord, &st->n_ord);
+            wave_append((fr & FR_LONG) != 0u, hpos, lng, &st->n_long);
         }
         // (b) my last run ends with my last item
         const uint64_t cl = c[IPL - 1];
         {
             const bool act = cl != NO_CELL && cl != cnext && hp != 0;
-            bool risky, lf, un;
-            finish_run<MULTI>(act, (uint32_t)(cl >> pk.bb), (uint32_t)(cl & pk.bmask),
-                              act ? p0 + IPL - (hp - 1u) : 0u, x, D, low_pool, high_pool, mask_pool, longbm,
-                              risky, lf, un);
+            const uint32_t fr = finish_run<MULTI>(act, (uint32_t)(cl >> pk.bb), (uint32_t)(cl & pk.bmask),
+                                                  act ? p0 + IPL - (hp - 1u) : 0u, x, D, low_pool, high_pool,
+                                                  mask_pool, longbm);
             const uint32_t hpos = (uint32_t)(hp - 1u);
-            wave_append(risky || un, hpos, ord, &st->n_ord);
-            wave_append(lf, hpos, lng, &st->n_long);
+            wave_append((fr & (FR_RISKY | FR_UNLINKED)) != 0u, hpos, ord, &st->n_ord);
+            wave_append((fr & FR_LONG) != 0u, hpos, lng, &st->n_long);
         }
         // (c) the group's last run continues past the window: one extension window
         const bool ext = __shfl((int)(cl != NO_CELL && cl == cnext && hp != 0), W - 1, W) != 0;
@@ -1074,21 +1079,21 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             add_words(e, sum);
             const uint64_t len = w0 + T + cnt - (he - 1u);
             const bool all_in = cnt == T;
-            bool risky = false, lf = false, un = false;
+            uint32_t fr = 0u;
             if (gl == 0) {
                 if (all_in) {
                     set_long<MULTI>(longbm, D, sid, b);
-                    lf = true;
-                    un = tables_of<MULTI>(sid, D, low_pool, high_pool, mask_pool).low[(size_t)b * 8u + 7u] <=
-                         QS_LINK_START;
+                    fr = FR_LONG | (tables_of<MULTI>(sid, D, low_pool, high_pool, mask_pool).low[(size_t)b * 8u + 7u] <=
+                                            QS_LINK_START
+                                        ? FR_UNLINKED
+                                        : 0u);
                 } else {
-                    finish_run<MULTI>(true, sid, b, len, e, D, low_pool, high_pool, mask_pool, longbm, risky, lf,
-                                      un);
+                    fr = finish_run<MULTI>(true, sid, b, len, e, D, low_pool, high_pool, mask_pool, longbm);
                 }
             }
             const uint32_t hpos = (uint32_t)(he - 1u);
-            wave_append(risky || un, hpos, ord, &st->n_ord);
-            wave_append(lf, hpos, lng, &st->n_long);
+            wave_append((fr & (FR_RISKY | FR_UNLINKED)) != 0u, hpos, ord, &st->n_ord);
+            wave_append((fr & FR_LONG) != 0u, hpos, lng, &st->n_long);
         }
     }
 }

@@ -738,8 +738,8 @@ extern "C" int qs_sf_create(uint64_t n, uint32_t m, uint32_t times, uint32_t see
     if (n < 1 || (m != 8 && m != 16 && m != 32) || times < 1 || times > 256 || seed_sz < 1 || n_masks < 1 ||
         n_masks > QS_MAX_MASKS || (key_bytes != 4 && key_bytes != 8))
         return QS_ERR_PARAM;
-    if (key_bytes == 4 && n > 0xffffffffull)
-        return QS_ERR_PARAM;
+    if (n > 0xffffffffull)
+        return QS_ERR_PARAM; // bucket indexes are 32-bit (0xffffffff marks "no bucket")
     if (key_bytes == 8 && seed_sz > 1)
         return QS_ERR_PARAM; // 64-bit keys sum the seeds in 64 bits (device_hash_function.h:6-17)
     if (n * m * (uint64_t)times > (1ull << 36))

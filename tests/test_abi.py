@@ -273,5 +273,5 @@ def test_sf_create_rejects_bad_parameters():
     h = C.c_void_p()
     for args in ((1021, 7, 3, 1, 1021, 4), (1021, 8, 0, 1, 1021, 4), (1021, 8, 3, 0, 1021, 4),
                  (1021, 8, 3, 1, 0, 4), (1021, 8, 3, 1, 5000, 4), (1021, 8, 3, 1, 1021, 2),
-                 (1021, 8, 3, 2, 1021, 8), (0, 8, 3, 1, 1021, 4)):
+                 (1021, 8, 3, 2, 1021, 8), (0, 8, 3, 1, 1021, 4), (1 << 32, 8, 1, 1, 1021, 8)):
         assert L.qs_sf_create(*args, 1, C.byref(h)) == q.QS_ERR_PARAM, args
