@@ -336,7 +336,7 @@ def main():
         "frac_of_random_32B_ceiling": round(out["query_gkeys_s"] / g32, 4) if out["query_gkeys_s"] else None,
         "ceiling_note": "2^25 random 32-byte reads from a 6 GiB buffer, 4-byte result each "
                         "(qs_calib_gather), measured in this run; every random read moves a whole "
-                        "128-byte line (profiles/calibration_copy_gather.md)"})
+                        "128-byte line (profiles/calibration_copy_fill_gather.md)"})
     if out["roofline"]["dominant"]:
         out["roofline"]["dominant"]["frac_of_copy_ceiling"] = round(out["roofline"]["dominant"]["achieved"] / copy_gbs, 4)
     out["roofline"]["copy_ceiling_gb_s"] = copy_gbs
@@ -479,6 +479,8 @@ def ceilings(reps=5, copy_gib=(1, 4), gathers=(16, 32, 64, 128)):
     kernels whose traffic is known (qsketch_amd/csrc/qs_calib.hip), timed
     with events on the stream they run on:
       * a streaming copy (read + write) of 1 GiB and of 4 GiB;
+      * a streaming fill (write only) of 4 GiB, with plain and with
+        non-temporal stores (the clear's pattern);
       * 2^25 random reads of 16, 32, 64 and 128 bytes from a 6 GiB buffer,
         each writing a 4-byte result (the query's pattern at 32 bytes)."""
     import numpy as np
@@ -500,13 +502,18 @@ def ceilings(reps=5, copy_gib=(1, 4), gathers=(16, 32, 64, 128)):
             ts.append(e0.elapsed_time(e1) * 1e-3)
         return float(np.median(ts))
 
-    out = {"copy": {}, "gather": {}}
+    out = {"copy": {}, "fill": {}, "gather": {}}
     for gib in copy_gib:
         src = torch.ones(gib << 28, dtype=torch.int32, device=dev)
         dst = torch.empty_like(src)
         t = timed(lambda: q.check(L.qs_calib_copy(src.data_ptr(), dst.data_ptr(), gib << 30, sp)))
         out["copy"][f"{gib}GiB"] = {"seconds": round(t, 6), "gb_s": round(2 * (gib << 30) / t / 1e9, 1)}
         del src, dst
+    dst = torch.empty(1 << 30, dtype=torch.int32, device=dev)  # 4 GiB
+    for nt in (0, 1):
+        t = timed(lambda: q.check(L.qs_calib_fill(dst.data_ptr(), 4 << 30, 0, nt, sp)))
+        out["fill"]["nontemporal" if nt else "plain"] = {"seconds": round(t, 6), "gb_s": round((4 << 30) / t / 1e9, 1)}
+    del dst
     if gathers:
         buf = torch.ones(6 << 28, dtype=torch.int32, device=dev)  # 6 GiB
         n = 1 << 25

@@ -19,6 +19,9 @@ extern "C" {
 
 /* Copy bytes (a multiple of 16) from d_src to d_dst, 16 bytes per lane. */
 int qs_calib_copy(const void *d_src, void *d_dst, uint64_t bytes, void *stream);
+/* Fill bytes (a multiple of 16) of d_dst with the 32-bit value, 16 bytes per
+ * lane; nontemporal != 0: non-temporal stores. */
+int qs_calib_fill(void *d_dst, uint64_t bytes, uint32_t value, int nontemporal, void *stream);
 /* n random reads of access_bytes (16, 32, 64 or 128) from the
  * access_bytes-aligned lines of d_buf (buf_bytes long), read i at the line
  * picked by a hash of (seed, i); the xor of each read goes to d_out[i]

@@ -168,6 +168,7 @@ _SIGS = {
     "qs_sf_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     # calibration kernels (include/qsketch_calib.h)
     "qs_calib_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "qs_calib_fill": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_void_p]),
     "qs_calib_gather": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p,
                                   C.c_void_p]),
     # generators (include/qsketch_gen.h)

@@ -2,6 +2,8 @@
 // sketch kernels run against and for calibrating the FETCH_SIZE / WRITE_SIZE
 // counters.  Measurement tools only: nothing in the sketch calls them.
 //   k_calib_copy    streaming copy, 16 bytes per lane per step
+//   k_calib_fill    streaming fill, 16 bytes per lane per step (plain or
+//                   non-temporal stores)
 //   k_calib_gather  n random reads of G bytes (G / 16 lanes, 16 bytes each)
 //                   from a buffer of G-byte lines; the group's xor goes to a
 //                   4-byte result per read, like a query's estimate
@@ -26,6 +28,19 @@ __device__ static inline uint64_t calib_mix(uint64_t seed, uint64_t i)
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+}
+
+template <bool NT>
+__global__ __launch_bounds__(256) void k_calib_fill(cu32x4 *__restrict__ dst, uint64_t n16, uint32_t value)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const cu32x4 v = {value, value, value, value};
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+        if (NT)
+            __builtin_nontemporal_store(v, dst + i);
+        else
+            dst[i] = v;
+    }
 }
 
 template <int LANES>
@@ -57,6 +72,22 @@ extern "C" int qs_calib_copy(const void *d_src, void *d_dst, uint64_t bytes, voi
         blocks = 256 * 64;
     hipLaunchKernelGGL(k_calib_copy, dim3((uint32_t)(blocks ? blocks : 1)), dim3(256), 0, (hipStream_t)stream,
                        (const cu32x4 *)d_src, (cu32x4 *)d_dst, n16);
+    return hipGetLastError() == hipSuccess ? QS_OK : QS_ERR_HIP;
+}
+
+extern "C" int qs_calib_fill(void *d_dst, uint64_t bytes, uint32_t value, int nontemporal, void *stream)
+{
+    if (!d_dst || bytes % 16)
+        return QS_ERR_PARAM;
+    const uint64_t n16 = bytes / 16;
+    uint64_t blocks = (n16 + 255) / 256;
+    if (blocks > 256 * 64)
+        blocks = 256 * 64;
+    const dim3 grid((uint32_t)(blocks ? blocks : 1)), block(256);
+    if (nontemporal)
+        hipLaunchKernelGGL(k_calib_fill<true>, grid, block, 0, (hipStream_t)stream, (cu32x4 *)d_dst, n16, value);
+    else
+        hipLaunchKernelGGL(k_calib_fill<false>, grid, block, 0, (hipStream_t)stream, (cu32x4 *)d_dst, n16, value);
     return hipGetLastError() == hipSuccess ? QS_OK : QS_ERR_HIP;
 }
 
