@@ -41,16 +41,18 @@ void qs_tim_begin(qs_sketch *s, qs_evpair &e, int kind, uint64_t keys, hipStream
     e.a = e.b = nullptr;
     if (!s->timing)
         return;
-    hipEventCreate(&e.a);
-    hipEventCreate(&e.b);
-    hipEventRecord(e.a, st);
+    // timing only: no system-scope fence at the records (the host reads the
+    // sketch's memory after its own synchronisation, never through these)
+    (void)hipEventCreateWithFlags(&e.a, hipEventDisableSystemFence);
+    (void)hipEventCreateWithFlags(&e.b, hipEventDisableSystemFence);
+    (void)hipEventRecord(e.a, st);
 }
 
 void qs_tim_end(qs_sketch *s, qs_evpair &e, hipStream_t st)
 {
     if (!s->timing || !e.a)
         return;
-    hipEventRecord(e.b, st);
+    (void)hipEventRecord(e.b, st);
     ((qs_timer *)s->tim)->pend.push_back(e);
 }
 
