@@ -1808,12 +1808,17 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
     const qs_sdesc d0 = s->h_desc[0];
     const void *in = keys;
     int out = 0;
+    qs_evpair ep; // the previous pass's scatter, ended by the next pass's first record
     for (int ps = 0; ps < passes; ++ps) {
         const int shift = dbits * ps;
         const int nshift = ps + 1 < passes ? shift + dbits : -1; // the scatter writes the next digit
         qs_evpair e0, e1, e2;
-        if (bc.timed)
-            qs_tim_begin(s, e0, QS_TK_SORT_UP, cn, stream);
+        if (bc.timed) {
+            if (ps == 0)
+                qs_tim_begin(s, e0, QS_TK_SORT_UP, cn, stream);
+            else
+                qs_tim_next(s, ep, e0, QS_TK_SORT_UP, cn, stream);
+        }
         if (ps == 0)
             hipLaunchKernelGGL((k_bs_up<KB, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
@@ -1821,16 +1826,12 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
         else
             hipLaunchKernelGGL(k_bs_up_dig, dim3((ntiles + UP_TILES - 1) / UP_TILES), dim3(OS_THREADS), 0, stream,
                                sc.dig, cn, ntiles, nbins, tcount);
-        if (bc.timed) {
-            qs_tim_end(s, e0, stream);
-            qs_tim_begin(s, e1, QS_TK_SORT_SCAN, cn, stream);
-        }
+        if (bc.timed)
+            qs_tim_next(s, e0, e1, QS_TK_SORT_SCAN, cn, stream);
         hipLaunchKernelGGL(k_hist_scan_local, dim3(ngrp), dim3(256), 0, stream, tcount, ntiles, nbins, gsum);
         hipLaunchKernelGGL(k_hist_scan_groups, dim3(nbins), dim3(1024), 0, stream, gsum, ngrp, nbins, totals);
-        if (bc.timed) {
-            qs_tim_end(s, e1, stream);
-            qs_tim_begin(s, e2, QS_TK_SORT_DOWN, cn, stream);
-        }
+        if (bc.timed)
+            qs_tim_next(s, e1, e2, QS_TK_SORT_DOWN, cn, stream);
         if (ps == 0)
             hipLaunchKernelGGL((k_bs_down<KB, true, MULTI>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
@@ -1839,8 +1840,12 @@ static int sort_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_t 
             hipLaunchKernelGGL((k_bs_down<KB, false, false>), dim3(ntiles), dim3(OS_THREADS), 0, stream, in, cn,
                                base, d0, s->d_desc, bc.d_offs, s->nsk, s->pk, shift, dmask, nbins, ntiles,
                                tcount, gsum, totals, sc.it[out], sc.dig, nshift, s->d_st);
-        if (bc.timed)
-            qs_tim_end(s, e2, stream);
+        if (bc.timed) {
+            if (ps + 1 == passes)
+                qs_tim_end(s, e2, stream);
+            else
+                ep = e2;
+        }
         in = sc.it[out];
         out ^= 1;
     }
@@ -1910,15 +1915,13 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
         qs_tim_begin(s, es, QS_TK_SEG, cn, stream);
     launch_seg<MULTI>(s, sc, IT, cn, stream);
     if (bc.timed) {
-        qs_tim_end(s, es, stream);
-        qs_tim_begin(s, eo, QS_TK_ORDERED, cn, stream);
+        qs_tim_next(s, es, eo, QS_TK_ORDERED, cn, stream);
     }
     hipLaunchKernelGGL(k_ordered, dim3(grid_cap(((uint64_t)cn + 255) / 256, 8)), dim3(256), 0, stream, IT, cn,
                        sc.ord, s->d_desc, s->d_low, s->d_masks, s->pk, s->d_longbm, sc.lng, sc.trig,
                        sc.trig_cap, s->nsk == 1 ? s->h_desc[0].n_masks : 0u, s->d_st);
     if (bc.timed) {
-        qs_tim_end(s, eo, stream);
-        qs_tim_begin(s, ea, QS_TK_ASSIGN, cn, stream);
+        qs_tim_next(s, eo, ea, QS_TK_ASSIGN, cn, stream);
     }
     hipLaunchKernelGGL(k_trig_keys, dim3(grid_cap(((uint64_t)sc.trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
                        sc.trig, s->d_st, sc.trig_cap, s->pk.ib, sc.tk[0], sc.tv[0]);
@@ -1932,8 +1935,7 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
                        sc.trig, sc.tk[tc], sc.tv[tc], sc.trig_cap, s->pk.ib, s->d_desc, s->d_low,
                        s->d_high, s->d_tail_of, s->d_next, s->d_st);
     if (bc.timed) {
-        qs_tim_end(s, ea, stream);
-        qs_tim_begin(s, et, QS_TK_TAIL, cn, stream);
+        qs_tim_next(s, ea, et, QS_TK_TAIL, cn, stream);
     }
     // k_segw windows under 64 items mark runs long that fit inside one step
     const bool short_runs = s->seg_w != 0 && s->seg_w * s->seg_ipl < 64u;

@@ -386,7 +386,11 @@ struct qs_evpair {
     hipEvent_t a, b;
     int kind; // QS_TK_* kernel group, or QS_TIMING_KERNELS for a chunk marker
     uint64_t keys;
+    bool shared_a; // a is the previous pair's b (one record ends one group and begins the next)
 };
 void qs_tim_begin(qs_sketch *s, qs_evpair &e, int kind, uint64_t keys, hipStream_t st);
 void qs_tim_end(qs_sketch *s, qs_evpair &e, hipStream_t st);
+// end `prev` and begin `next` with one event: for adjacent groups, with
+// nothing launched between (each record idles the GPU for microseconds)
+void qs_tim_next(qs_sketch *s, qs_evpair &prev, qs_evpair &next, int kind, uint64_t keys, hipStream_t st);
 

@@ -39,6 +39,7 @@ void qs_tim_begin(qs_sketch *s, qs_evpair &e, int kind, uint64_t keys, hipStream
     e.kind = kind;
     e.keys = keys;
     e.a = e.b = nullptr;
+    e.shared_a = false;
     if (!s->timing)
         return;
     // timing only: no system-scope fence at the records (the host reads the
@@ -56,6 +57,22 @@ void qs_tim_end(qs_sketch *s, qs_evpair &e, hipStream_t st)
     ((qs_timer *)s->tim)->pend.push_back(e);
 }
 
+void qs_tim_next(qs_sketch *s, qs_evpair &prev, qs_evpair &next, int kind, uint64_t keys, hipStream_t st)
+{
+    if (!s->timing || !prev.a) {
+        qs_tim_end(s, prev, st);
+        qs_tim_begin(s, next, kind, keys, st);
+        return;
+    }
+    next.kind = kind;
+    next.keys = keys;
+    (void)hipEventRecord(prev.b, st); // ends prev and begins next
+    ((qs_timer *)s->tim)->pend.push_back(prev);
+    next.a = prev.b;
+    next.shared_a = true;
+    (void)hipEventCreateWithFlags(&next.b, hipEventDisableSystemFence);
+}
+
 static void tim_resolve(qs_sketch *s)
 {
     if (!s->tim)
@@ -63,8 +80,8 @@ static void tim_resolve(qs_sketch *s)
     qs_timer *t = (qs_timer *)s->tim;
     for (auto &e : t->pend) {
         float ms = 0.f;
-        hipEventSynchronize(e.b);
-        hipEventElapsedTime(&ms, e.a, e.b);
+        (void)hipEventSynchronize(e.b);
+        (void)hipEventElapsedTime(&ms, e.a, e.b);
         if (e.kind >= 0 && e.kind < QS_TIMING_KERNELS) {
             s->tot.kernel_ms[e.kind] += ms;
             s->tot.kernel_launches[e.kind]++;
@@ -79,8 +96,13 @@ static void tim_resolve(qs_sketch *s)
         } else if (e.kind == QS_TIMING_KERNELS) {
             s->tot.insert_keys += e.keys; // marker: keys of one insert chunk
         }
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
+    }
+    // destroyed after every elapsed time is read: a shared event is one
+    // pair's b and the next pair's a, and goes with the first
+    for (auto &e : t->pend) {
+        if (!e.shared_a)
+            (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
     t->pend.clear();
 }
