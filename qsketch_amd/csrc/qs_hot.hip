@@ -67,6 +67,8 @@ struct qs_hot {
     uint32_t *sk[2], *sv[2];   // sample sort (keys, unused values)
     uint32_t *ck[2], *cv[2];   // candidate sorts
     uint32_t *hist;            // radix histograms of both sorts
+    uint32_t *cflag;           // candidate flags of the sorted sample, scanned in place (+ total)
+    uint32_t *cbsum;           // block sums of that scan (+ total)
     uint32_t *ctl;             // HC_* words
     uint32_t *h_ctl;           // pinned copy
     uint32_t *tkey;            // hot table: key per slot
@@ -140,33 +142,63 @@ __global__ void k_hot_sample(const uint32_t *__restrict__ keys, uint32_t n, uint
         ctl[i] = 0;
 }
 
-// a run end of the sorted sample that is at least HOT_MIN_RUN long: its key
-// and length become a candidate (unordered; sorted next)
-__global__ void k_hot_cand(const uint32_t *__restrict__ a, uint32_t n, uint32_t *__restrict__ ck,
-                           uint32_t *__restrict__ cv, uint32_t *__restrict__ ctl)
+// The first index of the run that ends at i in the sorted sample a (the run
+// is at least HOT_MIN_RUN long): gallop back from the run end, then bisect;
+// the loads stay near i.
+__device__ static inline uint32_t run_start(const uint32_t *__restrict__ a, uint32_t i)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
     const uint32_t k = a[i];
-    if (i + 1 < n && a[i + 1] == k)
-        return;
-    if (i + 1 < HOT_MIN_RUN || a[i + 1 - HOT_MIN_RUN] != k)
-        return;
-    uint32_t lo = 0, hi = i; // the first index holding k
-    while (lo < hi) {
+    uint32_t hi = i + 1u - HOT_MIN_RUN, lo = hi, step = HOT_MIN_RUN;
+    while (lo > 0) { // a[lo] < k or lo = 0 ends the gallop
+        lo = lo > step ? lo - step : 0u;
+        if (a[lo] < k)
+            break;
+        hi = lo;
+        step <<= 1;
+    }
+    if (a[lo] == k) // lo = 0 holds k: the run starts at 0
+        hi = lo;
+    while (lo + 1u < hi) { // a[lo] < k <= a[hi]
         const uint32_t mid = (lo + hi) >> 1;
         if (a[mid] < k)
-            lo = mid + 1;
+            lo = mid;
         else
             hi = mid;
     }
-    const uint32_t slot = atomicAdd(ctl + HC_NCAND, 1u);
-    ck[slot] = k;
-    cv[slot] = i - lo + 1;
+    return hi;
 }
 
-// candidates sorted by key -> (HOT_SAMPLE - count, key), so that a stable
+// whether i ends a run of the sorted sample that is at least HOT_MIN_RUN long
+__device__ static inline bool cand_end(const uint32_t *__restrict__ a, uint32_t n, uint32_t i)
+{
+    const uint32_t k = a[i];
+    return !(i + 1 < n && a[i + 1] == k) && i + 1 >= HOT_MIN_RUN && a[i + 1 - HOT_MIN_RUN] == k;
+}
+
+// flag[i] = 1 when i ends a candidate run (scanned next into offsets)
+__global__ void k_hot_cand_flag(const uint32_t *__restrict__ a, uint32_t n, uint32_t *__restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        flag[i] = cand_end(a, n, i) ? 1u : 0u;
+}
+
+// Each candidate run's key and length at its scanned offset (off: exclusive
+// offsets, total at off[n]), so the candidates come out in key order, with no
+// atomic on a shared counter.
+__global__ void k_hot_cand(const uint32_t *__restrict__ a, uint32_t n, const uint32_t *__restrict__ off,
+                           uint32_t *__restrict__ ck, uint32_t *__restrict__ cv, uint32_t *__restrict__ ctl)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+        ctl[HC_NCAND] = off[n];
+    if (i >= n || off[i + 1] == off[i])
+        return;
+    ck[off[i]] = a[i];
+    cv[off[i]] = i - run_start(a, i) + 1;
+}
+
+// candidates in key order -> (HOT_SAMPLE - count, key), so that a stable
 // sort of the first orders them by count, ties by key
 __global__ void k_hot_rekey(const uint32_t *__restrict__ ck, const uint32_t *__restrict__ cv,
                             uint32_t *__restrict__ k2, uint32_t *__restrict__ v2, const uint32_t *__restrict__ ctl)
@@ -510,6 +542,8 @@ void qs_hot_state_free(void *state)
         (void)hipFree(h->cv[k]);
     }
     (void)hipFree(h->hist);
+    (void)hipFree(h->cflag);
+    (void)hipFree(h->cbsum);
     (void)hipFree(h->ctl);
     (void)hipFree(h->tkey);
     (void)hipFree(h->tinfo);
@@ -553,6 +587,8 @@ static int hot_alloc(void **state, uint64_t rest)
                  hipMalloc(&h->ck[k], 4ull * HOT_CAND_MAX) == hipSuccess &&
                  hipMalloc(&h->cv[k], 4ull * HOT_CAND_MAX) == hipSuccess;
         ok = ok && hipMalloc(&h->hist, 4ull * qs_sort_hist_words(HOT_SAMPLE)) == hipSuccess &&
+             hipMalloc(&h->cflag, 4ull * (HOT_SAMPLE + 1)) == hipSuccess &&
+             hipMalloc(&h->cbsum, 4ull * (HOT_SAMPLE / SCAN_SPAN + 1)) == hipSuccess &&
              hipMalloc(&h->ctl, 4ull * HC_WORDS) == hipSuccess &&
              hipMalloc(&h->tkey, 4ull * HOT_SLOTS) == hipSuccess &&
              hipMalloc(&h->tinfo, 4ull * HOT_SLOTS) == hipSuccess &&
@@ -606,13 +642,17 @@ static int hot_candidates(qs_hot *h, const uint32_t *keys, hipStream_t stream, c
     const int r0 = qs_sort_pairs(h->sk, h->sv, h->hist, HOT_SAMPLE, 32, stream);
     if (r0 < 0)
         return QS_ERR_HIP;
-    hipLaunchKernelGGL(k_hot_cand, dim3(HOT_SAMPLE / 256), dim3(256), 0, stream, h->sk[r0], HOT_SAMPLE, h->ck[0],
-                       h->cv[0], h->ctl);
-    const int r1 = qs_sort_pairs_dev(h->ck, h->cv, h->hist, h->ctl + HC_NCAND, HOT_CAND_MAX, 32, stream);
-    if (r1 < 0)
-        return QS_ERR_HIP;
-    uint32_t *k2[2] = {h->ck[r1 ^ 1], h->ck[r1]}, *v2[2] = {h->cv[r1 ^ 1], h->cv[r1]};
-    hipLaunchKernelGGL(k_hot_rekey, dim3(HOT_CAND_MAX / 256), dim3(256), 0, stream, h->ck[r1], h->cv[r1], k2[0],
+    // candidates in key order: flags, a scan, each at its offset
+    const uint32_t snb = HOT_SAMPLE / SCAN_SPAN;
+    hipLaunchKernelGGL(k_hot_cand_flag, dim3(HOT_SAMPLE / 256), dim3(256), 0, stream, h->sk[r0], HOT_SAMPLE,
+                       h->cflag);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(snb), dim3(1024), 0, stream, h->cflag, HOT_SAMPLE, h->cbsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, h->cbsum, snb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(snb), dim3(1024), 0, stream, h->cflag, HOT_SAMPLE, h->cbsum, snb);
+    hipLaunchKernelGGL(k_hot_cand, dim3(HOT_SAMPLE / 256), dim3(256), 0, stream, h->sk[r0], HOT_SAMPLE, h->cflag,
+                       h->ck[0], h->cv[0], h->ctl);
+    uint32_t *k2[2] = {h->ck[1], h->ck[0]}, *v2[2] = {h->cv[1], h->cv[0]};
+    hipLaunchKernelGGL(k_hot_rekey, dim3(HOT_CAND_MAX / 256), dim3(256), 0, stream, h->ck[0], h->cv[0], k2[0],
                        v2[0], h->ctl);
     const int r2 = qs_sort_pairs_dev(k2, v2, h->hist, h->ctl + HC_NCAND, HOT_CAND_MAX, 21, stream);
     if (r2 < 0)
