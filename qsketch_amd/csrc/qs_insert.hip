@@ -621,13 +621,13 @@ __device__ static inline void mask_to_words(uint32_t M, uint32_t a[7])
         a[k] = (((M >> (4 * k)) & 0xfu) * 0x00204081u) & 0x01010101u;
 }
 
-// Owner of a complete segment (packed per-byte counts a[], each <= 128):
-// returns true when the segment must go to the in-order replay.
-__device__ static inline bool apply_segment(uint32_t *__restrict__ low, uint32_t *__restrict__ high,
-                                            uint32_t b, const uint32_t a[7])
+// Owner of a complete segment (packed per-byte counts a[], each <= 128),
+// whose bucket words x, y are already loaded: returns true when the segment
+// must go to the in-order replay.
+__device__ static inline bool apply_loaded(uint32_t *__restrict__ low, uint32_t *__restrict__ high, uint32_t b,
+                                           const uint32_t a[7], uint4 x, uint4 y)
 {
     uint4 *bp = reinterpret_cast<uint4 *>(low + (size_t)b * 8u);
-    uint4 x = bp[0], y = bp[1];
     uint32_t W[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
     uint32_t w = W[7];
     if (w > QS_LINK_START) {
@@ -673,6 +673,13 @@ __device__ static inline bool apply_segment(uint32_t *__restrict__ low, uint32_t
     bp[0] = x;
     bp[1] = y;
     return false;
+}
+
+__device__ static inline bool apply_segment(uint32_t *__restrict__ low, uint32_t *__restrict__ high,
+                                            uint32_t b, const uint32_t a[7])
+{
+    const uint4 *bp = reinterpret_cast<const uint4 *>(low + (size_t)b * 8u);
+    return apply_loaded(low, high, b, a, bp[0], bp[1]);
 }
 
 // Append `flag` lanes' values to a global list, one atomic per wavefront.
@@ -879,6 +886,40 @@ __device__ static inline uint32_t finish_run(bool act, uint32_t sid, uint32_t b,
     return apply_segment(tb.low, tb.high, b, a) ? FR_RISKY : 0u;
 }
 
+// The same finish in two phases, so that the bucket loads of all the runs a
+// lane finishes in one window are in flight together (the runs are distinct
+// cells, so no load depends on another run's store): load_run reads the
+// bucket's two 16-byte words, apply_run then does what finish_run does.
+template <bool MULTI>
+__device__ static inline void load_run(bool act, uint64_t cell, const qs_pack &pk, const qs_sdesc *__restrict__ D,
+                                       uint32_t *low_pool, uint32_t *high_pool, const uint32_t *mask_pool,
+                                       uint4 &x, uint4 &y)
+{
+    if (!act)
+        return;
+    const seg_tables tb = tables_of<MULTI>((uint32_t)(cell >> pk.bb), D, low_pool, high_pool, mask_pool);
+    const uint4 *bp = reinterpret_cast<const uint4 *>(tb.low + (size_t)(uint32_t)(cell & pk.bmask) * 8u);
+    x = bp[0];
+    y = bp[1];
+}
+
+template <bool MULTI>
+__device__ static inline uint32_t apply_run(bool act, uint64_t cell, uint64_t len, const uint32_t a[7], uint4 x,
+                                            uint4 y, const qs_pack &pk, const qs_sdesc *__restrict__ D,
+                                            uint32_t *low_pool, uint32_t *high_pool, const uint32_t *mask_pool,
+                                            uint32_t *longbm)
+{
+    if (!act)
+        return 0u;
+    const uint32_t sid = (uint32_t)(cell >> pk.bb), b = (uint32_t)(cell & pk.bmask);
+    if (len >= 256u) {
+        set_long<MULTI>(longbm, D, sid, b);
+        return FR_LONG | (y.w <= QS_LINK_START ? FR_UNLINKED : 0u);
+    }
+    const seg_tables tb = tables_of<MULTI>(sid, D, low_pool, high_pool, mask_pool);
+    return apply_loaded(tb.low, tb.high, b, a, x, y) ? FR_RISKY : 0u;
+}
+
 template <bool MULTI, int W, int IPL>
 __global__ __launch_bounds__(SEG_THREADS) void k_segw(
     const uint64_t *__restrict__ IT, uint32_t n, const qs_sdesc *__restrict__ D, uint32_t n_masks0,
@@ -901,10 +942,17 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
     // the items either side of it) are loaded while this window is processed.
     // The loads are unconditional, from clamped positions, so that the wait
     // for this window's items does not also wait for them.
-    uint64_t nit[IPL], nedge;
+    // Each lane also loads one of the W items after the window: a run that
+    // continues past the window end and stops within them is finished
+    // without another load.
+    uint64_t nit[IPL], nedge, nxit;
     const uint64_t last = (uint64_t)n - 1u;
     auto edge_pos = [&](uint64_t w) -> uint64_t {
         const uint64_t e = gl == 0 ? (w > 0 ? w - 1u : 0u) : w + T; // only lanes 0 and W - 1 use it
+        return e < last ? e : last;
+    };
+    auto ext_pos = [&](uint64_t w) -> uint64_t {
+        const uint64_t e = w + T + gl;
         return e < last ? e : last;
     };
     {
@@ -913,13 +961,14 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
         for (int k = 0; k < IPL; ++k)
             nit[k] = IT[p0 + k < last ? p0 + k : last];
         nedge = IT[edge_pos(w0)];
+        nxit = IT[ext_pos(w0)];
     }
     for (uint64_t w0 = gid * T; w0 < n; w0 += ngroups * T) {
         const uint64_t p0 = w0 + (uint64_t)gl * IPL;
         uint64_t c[IPL];
         uint32_t M[IPL];
         uint64_t itk[IPL];
-        const uint64_t edge = nedge;
+        const uint64_t edge = nedge, xit = nxit;
 #pragma unroll
         for (int k = 0; k < IPL; ++k)
             itk[k] = nit[k];
@@ -929,6 +978,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             for (int k = 0; k < IPL; ++k)
                 nit[k] = IT[p1 + k < last ? p1 + k : last];
             nedge = IT[edge_pos(w1)];
+            nxit = IT[ext_pos(w1)];
         }
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
@@ -953,8 +1003,18 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
         if ((__ballot(anyhead) & gmask) == 0)
             continue; // the window lies inside a run owned by an earlier window
 
-        // lane-local runs: first run [0, f1), interior runs, last run [rs, IPL)
-        uint32_t cur[7], af[7], tmp[7];
+        // lane-local runs: first run [0, f1), interior runs, last run [rs, IPL).
+        // An interior run [rs, k) with rs >= 1 ends at item k - 1 (1 <= k - 1
+        // <= IPL - 2); it is recorded in slot k - 2 and applied further down
+        // with the lane's other runs of this window.
+        constexpr int NI = IPL > 2 ? IPL - 2 : 1;
+        uint32_t cur[7], af[7], tmp[7], ia[NI][7];
+        uint64_t icell[NI];
+        uint32_t ipos[NI];
+        bool iact[NI];
+#pragma unroll
+        for (int j = 0; j < NI; ++j)
+            iact[j] = false;
         mask_to_words(M[0], cur);
 #pragma unroll
         for (int k = 0; k < 7; ++k)
@@ -963,12 +1023,15 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
 #pragma unroll
         for (int k = 1; k < IPL; ++k) {
             const bool bnd = c[k] != c[k - 1];
-            // an interior run [rs, k) with rs >= 1: its head is in this lane
-            const bool inner = bnd && rs > 0 && c[k - 1] != NO_CELL;
-            const uint32_t fr = finish_run<MULTI>(inner, (uint32_t)(c[k - 1] >> pk.bb),
-                                                  (uint32_t)(c[k - 1] & pk.bmask), 1u, cur, D, low_pool, high_pool,
-                                                  mask_pool, longbm);
-            wave_append((fr & FR_RISKY) != 0u, (uint32_t)(p0 + rs), ord, &st->n_ord);
+            if (k >= 2) { // (rs is 0 at k = 1: no interior run ends there)
+                const int j = k >= 2 ? k - 2 : 0;
+                iact[j] = bnd && rs > 0 && c[k - 1] != NO_CELL;
+                icell[j] = c[k - 1];
+                ipos[j] = (uint32_t)(p0 + rs);
+#pragma unroll
+                for (int e = 0; e < 7; ++e)
+                    ia[j][e] = cur[e];
+            }
             if (bnd && rs == 0) {
 #pragma unroll
                 for (int j = 0; j < 7; ++j)
@@ -1010,43 +1073,61 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             }
         }
         // (a) my first run ends inside this lane
-        {
-            uint32_t carry[7];
+        uint32_t carry[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            carry[k] = __shfl_up(x[k], 1, W);
+        uint64_t hprev = shfl64w<W>(hp, (int)gl - 1);
+        if (gl == 0)
+            hprev = 0;
+        const uint64_t head = head0 ? p0 + 1u : hprev;
+        const bool act_a = f1 < (uint32_t)IPL && c[0] != NO_CELL && head != 0;
+        if (!head0) {
 #pragma unroll
             for (int k = 0; k < 7; ++k)
-                carry[k] = __shfl_up(x[k], 1, W);
-            uint64_t hprev = shfl64w<W>(hp, (int)gl - 1);
-            if (gl == 0)
-                hprev = 0;
-            const uint64_t head = head0 ? p0 + 1u : hprev;
-            const bool act = f1 < (uint32_t)IPL && c[0] != NO_CELL && head != 0;
-            if (!head0) {
+                af[k] += carry[k];
+        }
+        // (b) my last run ends with my last item
+        const uint64_t cl = c[IPL - 1];
+        const bool act_b = cl != NO_CELL && cl != cnext && hp != 0;
+        // (c) the group's last run continues past the window: lane 0 of the
+        // group finishes it below
+        const bool ext = __shfl((int)(cl != NO_CELL && cl == cnext && hp != 0), W - 1, W) != 0;
+        const uint64_t ce = shfl64w<W>(cl, W - 1);
+        // the buckets of (a), the interior runs, (b) and (c), loaded together
+        uint4 xa, ya, xb, yb, xe, ye, xi[NI], yi[NI];
+        load_run<MULTI>(ext && gl == 0, ce, pk, D, low_pool, high_pool, mask_pool, xe, ye);
+        load_run<MULTI>(act_a, c[0], pk, D, low_pool, high_pool, mask_pool, xa, ya);
 #pragma unroll
-                for (int k = 0; k < 7; ++k)
-                    af[k] += carry[k];
-            }
-            const uint32_t fr = finish_run<MULTI>(act, (uint32_t)(c[0] >> pk.bb), (uint32_t)(c[0] & pk.bmask),
-                                                  act ? p0 + f1 - (head - 1u) : 0u, af, D, low_pool, high_pool,
-                                                  mask_pool, longbm);
+        for (int j = 0; j < NI; ++j)
+            load_run<MULTI>(iact[j], icell[j], pk, D, low_pool, high_pool, mask_pool, xi[j], yi[j]);
+        load_run<MULTI>(act_b, cl, pk, D, low_pool, high_pool, mask_pool, xb, yb);
+        {
+            const uint32_t fr = apply_run<MULTI>(act_a, c[0], act_a ? p0 + f1 - (head - 1u) : 0u, af, xa, ya, pk, D,
+                                                 low_pool, high_pool, mask_pool, longbm);
             const uint32_t hpos = (uint32_t)(head - 1u);
             wave_append((fr & (FR_RISKY | FR_UNLINKED)) != 0u, hpos, ord, &st->n_ord);
             wave_append((fr & FR_LONG) != 0u, hpos, lng, &st->n_long);
         }
-        // (b) my last run ends with my last item
-        const uint64_t cl = c[IPL - 1];
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            // an interior run is shorter than IPL items: never long
+            const uint32_t fr = apply_run<MULTI>(iact[j], icell[j], 1u, ia[j], xi[j], yi[j], pk, D, low_pool,
+                                                 high_pool, mask_pool, longbm);
+            wave_append((fr & FR_RISKY) != 0u, ipos[j], ord, &st->n_ord);
+        }
         {
-            const bool act = cl != NO_CELL && cl != cnext && hp != 0;
-            const uint32_t fr = finish_run<MULTI>(act, (uint32_t)(cl >> pk.bb), (uint32_t)(cl & pk.bmask),
-                                                  act ? p0 + IPL - (hp - 1u) : 0u, x, D, low_pool, high_pool,
-                                                  mask_pool, longbm);
+            const uint32_t fr = apply_run<MULTI>(act_b, cl, act_b ? p0 + IPL - (hp - 1u) : 0u, x, xb, yb, pk, D,
+                                                 low_pool, high_pool, mask_pool, longbm);
             const uint32_t hpos = (uint32_t)(hp - 1u);
             wave_append((fr & (FR_RISKY | FR_UNLINKED)) != 0u, hpos, ord, &st->n_ord);
             wave_append((fr & FR_LONG) != 0u, hpos, lng, &st->n_long);
         }
-        // (c) the group's last run continues past the window: one extension window
-        const bool ext = __shfl((int)(cl != NO_CELL && cl == cnext && hp != 0), W - 1, W) != 0;
+        // (c): the run's items in the extension window [w0 + T, w0 + 2T) are
+        // a prefix of it (the items are sorted by cell).  When fewer than W
+        // of the W items loaded after the window belong to the run, they are
+        // all of it; otherwise the extension window is read.
         if (ext) {
-            const uint64_t ce = shfl64w<W>(cl, W - 1);
             const uint64_t he = shfl64w<W>(hp, W - 1);
             uint32_t e[7];
 #pragma unroll
@@ -1057,15 +1138,24 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
 #pragma unroll
             for (int k = 0; k < 7; ++k)
                 sum[k] = 0;
+            const bool in1 = w0 + T + gl < n && it_cell(xit, pk) == ce;
+            if ((__ballot(in1) & gmask) != gmask) {
+                if (in1) {
+                    mask_to_words(mask_of<MULTI>(sid, item_mu<MULTI>(xit, pk, D), D, mask_pool, s_masks, pk.unm), sum);
+                    cnt = 1;
+                }
+            } else {
 #pragma unroll
-            for (int k = 0; k < IPL; ++k) {
-                const uint64_t q = w0 + T + (uint64_t)gl * IPL + k;
-                if (q < n) {
-                    const uint64_t iq = IT[q];
-                    if (it_cell(iq, pk) == ce) {
-                        mask_to_words(mask_of<MULTI>(sid, item_mu<MULTI>(iq, pk, D), D, mask_pool, s_masks, pk.unm), tmp);
-                        add_words(sum, tmp);
-                        ++cnt;
+                for (int k = 0; k < IPL; ++k) {
+                    const uint64_t q = w0 + T + (uint64_t)gl * IPL + k;
+                    if (q < n) {
+                        const uint64_t iq = IT[q];
+                        if (it_cell(iq, pk) == ce) {
+                            mask_to_words(mask_of<MULTI>(sid, item_mu<MULTI>(iq, pk, D), D, mask_pool, s_masks, pk.unm),
+                                          tmp);
+                            add_words(sum, tmp);
+                            ++cnt;
+                        }
                     }
                 }
             }
@@ -1083,12 +1173,9 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             if (gl == 0) {
                 if (all_in) {
                     set_long<MULTI>(longbm, D, sid, b);
-                    fr = FR_LONG | (tables_of<MULTI>(sid, D, low_pool, high_pool, mask_pool).low[(size_t)b * 8u + 7u] <=
-                                            QS_LINK_START
-                                        ? FR_UNLINKED
-                                        : 0u);
+                    fr = FR_LONG | (ye.w <= QS_LINK_START ? FR_UNLINKED : 0u);
                 } else {
-                    fr = finish_run<MULTI>(true, sid, b, len, e, D, low_pool, high_pool, mask_pool, longbm);
+                    fr = apply_run<MULTI>(true, ce, len, e, xe, ye, pk, D, low_pool, high_pool, mask_pool, longbm);
                 }
             }
             const uint32_t hpos = (uint32_t)(he - 1u);
