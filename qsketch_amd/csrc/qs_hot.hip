@@ -7,7 +7,8 @@
 // v3:160-190).  So a long insert call runs as:
 //   1. the candidates: the keys occurring at least HOT_MIN_RUN times among the
 //      first HOT_SAMPLE keys (a sort of that sample, then its runs), ordered
-//      by count (ties by key) and capped at HOT_MAX;
+//      by count (ties by key) and capped at HOT_MAX; on a second stream, at
+//      the same time as
 //   2. the first HOT_PREFIX keys through the ordinary pipeline (qs_insert.hip),
 //      which links the buckets of the hot keys;
 //   3. the hot set: the candidates whose bucket is linked now, in a table of
@@ -18,14 +19,14 @@
 //   4. the rest of the call in one pass: a hot key is counted in LDS (per
 //      persistent block, summed over blocks at the end and added to its high
 //      row once), every other key is written, in stream order, to a cold
-//      array;
+//      array (packed per chunk, then moved together after a scan);
 //   5. the cold array through the ordinary pipeline.
 // The result is the sequential one: the hot keys' inserts commute with
 // everything after their bucket is linked, and the cold keys keep their
 // stream order (so the allocation ids, given in stream order of the
 // promoting inserts, are unchanged).  A call whose sample shows no skew
-// (candidates covering less than HOT_MIN_COVER of it) runs the ordinary
-// pipeline unchanged.
+// (candidates covering less than HOT_MIN_COVER of it) runs the rest of the
+// call after the prefix through the ordinary pipeline.
 //
 // qs_hot_split_keys serves a sink whose inserts all commute (the SF sketch,
 // qs_sf.hip): no prefix and no link condition; every candidate enters the
@@ -33,7 +34,6 @@
 #include "qs_internal.h"
 
 #include <cstring>
-#include <functional>
 
 #define CK(x)                                                                                    \
     do {                                                                                         \
@@ -78,12 +78,16 @@ struct qs_hot {
     uint32_t *part;            // per persistent block and slot: hot inserts counted
     uint32_t nblk;             // persistent blocks (rows of part)
     uint32_t *ccold;           // cold keys per chunk, scanned in place (+ total)
-    uint64_t *cmask;           // per chunk and row: the ballot of the cold keys
+    uint32_t *stage;           // per chunk of HOT_CHUNK slots: its cold keys, packed at the front
     uint32_t *bsum;            // block sums of the chunk-count scan (+ total)
     uint64_t chunk_cap;
     uint32_t *cold;            // the cold keys of the call, in stream order
     uint64_t cold_cap;
     uint32_t seq;              // sequence number of the last publication to h_ctl
+    const uint32_t *cand;      // the candidates, most frequent first (set by hot_cand_launch)
+    hipStream_t side;          // the candidate phase of a plain insert runs here (default priority)
+    hipEvent_t ev_fork;        // the caller's stream up to the call
+    hipEvent_t ev_cand;        // the candidate phase done
 };
 
 // The host reads a few device words without a copy or a synchronisation
@@ -329,14 +333,15 @@ __device__ static inline int hot_find(const uint4 *skey4, uint32_t k)
 
 // Pass 1 over the keys after the prefix.  A wavefront owns chunks of
 // HOT_CHUNK consecutive keys (HOT_IPT rows of 64), loads the next chunk while
-// it works on this one, counts hot keys per slot in LDS and records, per row,
-// the ballot of its cold keys and, per chunk, their number.  Persistent
-// blocks (one per CU: the two LDS arrays take 128 KB) take chunks in turn
-// and write their row of per-slot counts at the end.
+// it works on this one, counts hot keys per slot in LDS and writes the
+// chunk's cold keys, in stream order, to the front of the chunk's own
+// HOT_CHUNK slots of `stage`, and their number to ccold.  Persistent blocks
+// (one per CU: the two LDS arrays take 128 KB) take chunks in turn and write
+// their row of per-slot counts at the end.
 __global__ __launch_bounds__(HOT_THREADS) void k_hot_count(const uint32_t *__restrict__ keys, uint64_t n,
                                                            uint32_t nchunks, const uint32_t *__restrict__ tkey,
                                                            uint32_t *__restrict__ part,
-                                                           uint64_t *__restrict__ cmask,
+                                                           uint32_t *__restrict__ stage,
                                                            uint32_t *__restrict__ ccold)
 {
     __shared__ uint4 skey4[HOT_BUCKETS];
@@ -347,6 +352,7 @@ __global__ __launch_bounds__(HOT_THREADS) void k_hot_count(const uint32_t *__res
         scnt[e] = 0;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t nw = gridDim.x * (HOT_THREADS / 64);
     uint32_t c = blockIdx.x * (HOT_THREADS / 64) + (threadIdx.x >> 6);
     uint32_t k[HOT_IPT], kn[HOT_IPT];
@@ -363,16 +369,18 @@ __global__ __launch_bounds__(HOT_THREADS) void k_hot_count(const uint32_t *__res
             kn[r] = cn < nchunks && i < n ? __builtin_nontemporal_load(keys + i) : HOT_EMPTY;
         }
         uint32_t cold = 0;
+        uint32_t *dst = stage + (size_t)c * HOT_CHUNK;
 #pragma unroll
         for (uint32_t r = 0; r < HOT_IPT; ++r) {
             const uint64_t i = (uint64_t)c * HOT_CHUNK + r * 64u + lane;
             const int s = hot_find(skey4, k[r]); // the key past the end is HOT_EMPTY: -1
             if (s >= 0)
                 atomicAdd(&scnt[s], 1u);
-            const uint64_t m = __ballot((i < n) & (s < 0)); // every lane, outside any branch
+            const bool is_cold = (i < n) & (s < 0);
+            const uint64_t m = __ballot(is_cold); // every lane, outside any branch
+            if (is_cold)
+                dst[cold + (uint32_t)__popcll(m & below)] = k[r];
             cold += (uint32_t)__popcll(m);
-            if (lane == 0)
-                cmask[(size_t)c * HOT_IPT + r] = m;
         }
         if (lane == 0)
             ccold[c] = cold;
@@ -385,32 +393,25 @@ __global__ __launch_bounds__(HOT_THREADS) void k_hot_count(const uint32_t *__res
         part[(size_t)blockIdx.x * HOT_SLOTS + e] = scnt[e];
 }
 
-// Pass 2: every chunk writes its cold keys at its scanned offset, in stream
-// order, from the ballots of pass 1 (no table, no LDS).
-__global__ __launch_bounds__(256) void k_hot_compact(const uint32_t *__restrict__ keys, uint64_t n,
-                                                     uint32_t nchunks, const uint64_t *__restrict__ cmask,
-                                                     const uint32_t *__restrict__ coff,
-                                                     uint32_t *__restrict__ cold)
+// Pass 2: every chunk moves its packed cold keys from `stage` to its scanned
+// offset (coff[c] .. coff[c + 1]), so the cold keys end up contiguous in
+// stream order.  Only the cold keys are read again, not the whole stream.
+__global__ __launch_bounds__(256) void k_hot_move(const uint32_t *__restrict__ stage, uint32_t nchunks,
+                                                  const uint32_t *__restrict__ coff, uint32_t *__restrict__ cold)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t nw = gridDim.x * 4u;
     for (uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6); c < nchunks; c += nw) {
-        uint32_t k[HOT_IPT];
-        uint64_t m[HOT_IPT];
+        const uint32_t off = coff[c], cnt = coff[c + 1] - off; // cnt <= HOT_CHUNK
+        const uint32_t *src = stage + (size_t)c * HOT_CHUNK;
+        uint32_t v[HOT_IPT];
 #pragma unroll
-        for (uint32_t r = 0; r < HOT_IPT; ++r) {
-            const uint64_t i = (uint64_t)c * HOT_CHUNK + r * 64u + lane;
-            m[r] = cmask[(size_t)c * HOT_IPT + r];
-            k[r] = i < n ? __builtin_nontemporal_load(keys + i) : 0u;
-        }
-        uint32_t off = coff[c];
+        for (uint32_t r = 0; r < HOT_IPT; ++r)
+            v[r] = r * 64u + lane < cnt ? __builtin_nontemporal_load(src + r * 64u + lane) : 0u;
 #pragma unroll
-        for (uint32_t r = 0; r < HOT_IPT; ++r) {
-            if ((m[r] >> lane) & 1ull)
-                cold[off + (uint32_t)__popcll(m[r] & below)] = k[r];
-            off += (uint32_t)__popcll(m[r]);
-        }
+        for (uint32_t r = 0; r < HOT_IPT; ++r)
+            if (r * 64u + lane < cnt)
+                cold[(size_t)off + r * 64u + lane] = v[r];
     }
 }
 
@@ -551,11 +552,17 @@ void qs_hot_state_free(void *state)
     (void)hipFree(h->tfill);
     (void)hipFree(h->part);
     (void)hipFree(h->ccold);
-    (void)hipFree(h->cmask);
+    (void)hipFree(h->stage);
     (void)hipFree(h->bsum);
     (void)hipFree(h->cold);
     if (h->h_ctl)
         (void)hipHostFree(h->h_ctl);
+    if (h->ev_fork)
+        (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_cand)
+        (void)hipEventDestroy(h->ev_cand);
+    if (h->side)
+        (void)hipStreamDestroy(h->side);
     free(h);
 }
 
@@ -595,7 +602,10 @@ static int hot_alloc(void **state, uint64_t rest)
              hipMalloc(&h->tmask, 4ull * HOT_SLOTS) == hipSuccess &&
              hipMalloc(&h->tfill, 4ull * HOT_BUCKETS) == hipSuccess &&
              hipMalloc(&h->part, 4ull * HOT_SLOTS * h->nblk) == hipSuccess &&
-             hipHostMalloc(&h->h_ctl, 4ull * HOT_PUB_WORDS, hipHostMallocCoherent) == hipSuccess;
+             hipHostMalloc(&h->h_ctl, 4ull * HOT_PUB_WORDS, hipHostMallocCoherent) == hipSuccess &&
+             hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&h->ev_cand, hipEventDisableTiming) == hipSuccess;
         if (ok) {
             memset(h->h_ctl, 0, 4ull * HOT_PUB_WORDS);
             h->seq = 0;
@@ -606,14 +616,14 @@ static int hot_alloc(void **state, uint64_t rest)
     const uint64_t chunks = (rest + HOT_CHUNK - 1) / HOT_CHUNK;
     if (h->chunk_cap < chunks) {
         (void)hipFree(h->ccold);
-        (void)hipFree(h->cmask);
+        (void)hipFree(h->stage);
         (void)hipFree(h->bsum);
         h->ccold = nullptr;
-        h->cmask = nullptr;
+        h->stage = nullptr;
         h->bsum = nullptr;
         h->chunk_cap = 0;
         if (hipMalloc(&h->ccold, 4ull * (chunks + 1)) != hipSuccess ||
-            hipMalloc(&h->cmask, 8ull * HOT_IPT * chunks) != hipSuccess ||
+            hipMalloc(&h->stage, 4ull * HOT_CHUNK * chunks) != hipSuccess ||
             hipMalloc(&h->bsum, 4ull * ((chunks + SCAN_SPAN - 1) / SCAN_SPAN + 1)) != hipSuccess)
             return QS_ERR_NOMEM;
         h->chunk_cap = chunks;
@@ -629,14 +639,11 @@ static int hot_alloc(void **state, uint64_t rest)
     return QS_OK;
 }
 
-// Candidates from the first HOT_SAMPLE keys (step 1): *cand = the candidate
-// keys, most frequent first (at most HOT_MAX are used); *skewed = whether they
-// cover at least HOT_MIN_COVER of the sample.  `launched` (may be empty) runs
-// once the kernels are queued, before the wait for the cover.
-static int hot_candidates(qs_hot *h, const uint32_t *keys, hipStream_t stream, const std::function<void()> &launched,
-                          const uint32_t **cand, bool *skewed)
+// Candidates from the first HOT_SAMPLE keys (step 1), queued on `stream`:
+// h->cand = the candidate keys, most frequent first (at most HOT_MAX are
+// used).  hot_cand_result then waits for their cover.
+static int hot_cand_launch(qs_hot *h, const uint32_t *keys, hipStream_t stream)
 {
-    *skewed = false;
     hipLaunchKernelGGL(k_hot_sample, dim3(HOT_SAMPLE / 256), dim3(256), 0, stream, keys, HOT_SAMPLE, h->sk[0],
                        h->sv[0], h->ctl);
     const int r0 = qs_sort_pairs(h->sk, h->sv, h->hist, HOT_SAMPLE, 32, stream);
@@ -659,14 +666,20 @@ static int hot_candidates(qs_hot *h, const uint32_t *keys, hipStream_t stream, c
         return QS_ERR_HIP;
     hipLaunchKernelGGL(k_hot_cover, dim3(1), dim3(1024), 0, stream, k2[r2], h->ctl);
     CK(hipGetLastError());
-    if (launched)
-        launched();
+    h->cand = v2[r2];
+    return QS_OK;
+}
+
+// *skewed = whether the candidates cover at least HOT_MIN_COVER of the
+// sample (read after the candidate kernels queued on `stream`)
+static int hot_cand_result(qs_hot *h, hipStream_t stream, bool *skewed)
+{
+    *skewed = false;
     const int rc = hot_read(h, h->ctl, HC_WORDS, stream);
     if (rc)
         return rc;
     const uint64_t cover = h->h_ctl[HC_COVER];
     *skewed = h->h_ctl[HC_NCAND] != 0 && cover * HOT_MIN_COVER_DEN >= (uint64_t)HOT_SAMPLE * HOT_MIN_COVER_NUM;
-    *cand = v2[r2];
     return QS_OK;
 }
 
@@ -685,26 +698,13 @@ int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stre
         return rc;
     qs_hot *h = (qs_hot *)s->hot;
     const qs_sdesc d0 = s->h_desc[0];
-    qs_evpair e0, e1, e2;
+    qs_evpair e1, e2;
     const bool timed = s->timing != 0;
 
-    // 1. candidates from the head of the call
-    if (timed)
-        qs_tim_begin(s, e0, QS_TK_HOT, 0, stream);
-    const uint32_t *cand = nullptr;
-    bool skewed = false;
-    const std::function<void()> launched = timed ? std::function<void()>([&]() { qs_tim_end(s, e0, stream); })
-                                                 : std::function<void()>();
-    rc = hot_candidates(h, keys, stream, launched, &cand, &skewed);
-    if (rc)
-        return rc;
-    if (!skewed)
-        return QS_OK; // not skewed enough to pay for the filter
-
-    // 2. the prefix the ordinary way.  The insert scratch is sized now for
-    // every key after the prefix, the most the cold part can hold: the cold
-    // count varies a little between calls (the hot table's placement races),
-    // and growing the scratch between launches would free and allocate.
+    // The insert scratch is sized first for every key after the prefix, the
+    // most the cold part can hold: the cold count varies a little between
+    // calls (the hot table's placement races), and growing the scratch
+    // between launches would free and allocate.
     {
         const uint64_t offs_r[2] = {0, rest};
         const uint64_t cap = qs_chunk_keys(s, offs_r);
@@ -712,27 +712,51 @@ int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stre
         if (rc)
             return rc;
     }
+    // 2. the prefix goes through the ordinary pipeline on the caller's
+    // stream while 1. the candidates run on the side stream, from the point
+    // the caller's stream had reached at this call (the two share no memory
+    // but the keys, which both only read).  The prefix is queued first: its
+    // launches are what keeps the GPU busy while the host queues the
+    // candidates' short kernels.  The prefix is inserted whatever the sample
+    // shows: a prefix and then the rest is the sequential result, as for any
+    // call split into chunks.
+    CK(hipEventRecord(h->ev_fork, stream));
     *used = 1;
     const uint64_t offs_p[2] = {0, HOT_PREFIX};
     rc = qs_insert_set(s, keys, offs_p, nullptr, stream, false, false);
     if (rc)
         return rc;
+    CK(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    rc = hot_cand_launch(h, keys, h->side);
+    if (rc)
+        return rc;
+    CK(hipEventRecord(h->ev_cand, h->side));
+    bool skewed = false;
+    rc = hot_cand_result(h, h->side, &skewed); // the GPU runs the prefix meanwhile
+    if (rc)
+        return rc;
+    if (!skewed) { // not skewed enough to pay for the filter: the rest the ordinary way
+        CK(hipStreamWaitEvent(stream, h->ev_cand, 0));
+        const uint64_t offs_r[2] = {0, rest};
+        return qs_insert_set(s, keys + HOT_PREFIX, offs_r, nullptr, stream, true, false);
+    }
+    const uint32_t *cand = h->cand;
 
     // 3. the hot set, 4. the filter
     const uint32_t nchunks = (uint32_t)((rest + HOT_CHUNK - 1) / HOT_CHUNK);
     if (timed)
-        qs_tim_begin(s, e1, QS_TK_HOT, 0, stream);
+        qs_tim_begin(s, e1, QS_TK_HOT, 0, stream); // a wait for the candidates counts here
+    CK(hipStreamWaitEvent(stream, h->ev_cand, 0));
     hipLaunchKernelGGL(k_hot_clear, dim3(HOT_SLOTS / 256), dim3(256), 0, stream, h->tkey, h->tfill, h->ctl);
     hipLaunchKernelGGL(k_hot_build, dim3(HOT_MAX / 256), dim3(256), 0, stream, cand, d0, s->d_low, s->d_masks,
                        h->tkey, h->tinfo, h->tmask, h->tfill, h->ctl);
     hipLaunchKernelGGL(k_hot_count, dim3(h->nblk), dim3(HOT_THREADS), 0, stream, keys + HOT_PREFIX, rest,
-                       nchunks, h->tkey, h->part, h->cmask, h->ccold);
+                       nchunks, h->tkey, h->part, h->stage, h->ccold);
     const uint32_t nb = (nchunks + SCAN_SPAN - 1) / SCAN_SPAN; // <= 8192 for any u32 chunk count of a call
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(1024), 0, stream, h->ccold, nchunks, h->bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, h->bsum, nb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, stream, h->ccold, nchunks, h->bsum, nb);
-    hipLaunchKernelGGL(k_hot_compact, dim3(h->nblk * 8u), dim3(256), 0, stream, keys + HOT_PREFIX, rest, nchunks,
-                       h->cmask, h->ccold, h->cold);
+    hipLaunchKernelGGL(k_hot_move, dim3(h->nblk * 8u), dim3(256), 0, stream, h->stage, nchunks, h->ccold, h->cold);
     hipLaunchKernelGGL(k_hot_flush, dim3(HOT_SLOTS / 64), dim3(256), 0, stream, h->part, h->nblk, h->tkey,
                        h->tinfo, h->tmask, s->d_high);
     CK(hipGetLastError());
@@ -777,22 +801,24 @@ int qs_hot_split_keys(void **state, const uint32_t *keys, uint64_t n, hipStream_
     if (rc)
         return rc;
     qs_hot *h = (qs_hot *)*state;
-    const uint32_t *cand = nullptr;
     bool skewed = false;
-    rc = hot_candidates(h, keys, stream, std::function<void()>(), &cand, &skewed);
+    rc = hot_cand_launch(h, keys, stream);
+    if (rc)
+        return rc;
+    rc = hot_cand_result(h, stream, &skewed);
     if (rc || !skewed)
         return rc;
+    const uint32_t *cand = h->cand;
     const uint32_t nchunks = (uint32_t)((n + HOT_CHUNK - 1) / HOT_CHUNK);
     hipLaunchKernelGGL(k_hot_clear, dim3(HOT_SLOTS / 256), dim3(256), 0, stream, h->tkey, h->tfill, h->ctl);
     hipLaunchKernelGGL(k_hot_build_any, dim3(HOT_MAX / 256), dim3(256), 0, stream, cand, h->tkey, h->tfill, h->ctl);
     hipLaunchKernelGGL(k_hot_count, dim3(h->nblk), dim3(HOT_THREADS), 0, stream, keys, n, nchunks, h->tkey, h->part,
-                       h->cmask, h->ccold);
+                       h->stage, h->ccold);
     const uint32_t nb = (nchunks + SCAN_SPAN - 1) / SCAN_SPAN;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(1024), 0, stream, h->ccold, nchunks, h->bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, h->bsum, nb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, stream, h->ccold, nchunks, h->bsum, nb);
-    hipLaunchKernelGGL(k_hot_compact, dim3(h->nblk * 8u), dim3(256), 0, stream, keys, n, nchunks, h->cmask, h->ccold,
-                       h->cold);
+    hipLaunchKernelGGL(k_hot_move, dim3(h->nblk * 8u), dim3(256), 0, stream, h->stage, nchunks, h->ccold, h->cold);
     CK(hipGetLastError());
     rc = hot_read(h, h->ccold + nchunks, 1, stream);
     if (rc)
