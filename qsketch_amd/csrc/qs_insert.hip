@@ -920,8 +920,10 @@ __device__ static inline uint32_t apply_run(bool act, uint64_t cell, uint64_t le
     return apply_loaded(tb.low, tb.high, b, a, x, y) ? FR_RISKY : 0u;
 }
 
+// (A single sketch's shapes of up to 4 items per lane are held to 168
+// registers, three wavefronts per SIMD, which they fit without spilling.)
 template <bool MULTI, int W, int IPL>
-__global__ __launch_bounds__(SEG_THREADS) void k_segw(
+__global__ __launch_bounds__(SEG_THREADS) __attribute__((amdgpu_waves_per_eu(!MULTI && IPL <= 4 ? 3 : 1))) void k_segw(
     const uint64_t *__restrict__ IT, uint32_t n, const qs_sdesc *__restrict__ D, uint32_t n_masks0,
     uint32_t *__restrict__ low_pool, uint32_t *__restrict__ high_pool,
     const uint32_t *__restrict__ mask_pool, qs_pack pk, uint32_t *__restrict__ longbm,
@@ -1102,31 +1104,20 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
         for (int j = 0; j < NI; ++j)
             load_run<MULTI>(iact[j], icell[j], pk, D, low_pool, high_pool, mask_pool, xi[j], yi[j]);
         load_run<MULTI>(act_b, cl, pk, D, low_pool, high_pool, mask_pool, xb, yb);
-        {
-            const uint32_t fr = apply_run<MULTI>(act_a, c[0], act_a ? p0 + f1 - (head - 1u) : 0u, af, xa, ya, pk, D,
-                                                 low_pool, high_pool, mask_pool, longbm);
-            const uint32_t hpos = (uint32_t)(head - 1u);
-            wave_append((fr & (FR_RISKY | FR_UNLINKED)) != 0u, hpos, ord, &st->n_ord);
-            wave_append((fr & FR_LONG) != 0u, hpos, lng, &st->n_long);
-        }
+        const uint32_t fr_a = apply_run<MULTI>(act_a, c[0], act_a ? p0 + f1 - (head - 1u) : 0u, af, xa, ya, pk, D,
+                                               low_pool, high_pool, mask_pool, longbm);
+        uint32_t fr_i[NI];
 #pragma unroll
-        for (int j = 0; j < NI; ++j) {
-            // an interior run is shorter than IPL items: never long
-            const uint32_t fr = apply_run<MULTI>(iact[j], icell[j], 1u, ia[j], xi[j], yi[j], pk, D, low_pool,
-                                                 high_pool, mask_pool, longbm);
-            wave_append((fr & FR_RISKY) != 0u, ipos[j], ord, &st->n_ord);
-        }
-        {
-            const uint32_t fr = apply_run<MULTI>(act_b, cl, act_b ? p0 + IPL - (hp - 1u) : 0u, x, xb, yb, pk, D,
-                                                 low_pool, high_pool, mask_pool, longbm);
-            const uint32_t hpos = (uint32_t)(hp - 1u);
-            wave_append((fr & (FR_RISKY | FR_UNLINKED)) != 0u, hpos, ord, &st->n_ord);
-            wave_append((fr & FR_LONG) != 0u, hpos, lng, &st->n_long);
-        }
+        for (int j = 0; j < NI; ++j) // an interior run is shorter than IPL items: never long
+            fr_i[j] = apply_run<MULTI>(iact[j], icell[j], 1u, ia[j], xi[j], yi[j], pk, D, low_pool, high_pool,
+                                       mask_pool, longbm);
+        const uint32_t fr_b = apply_run<MULTI>(act_b, cl, act_b ? p0 + IPL - (hp - 1u) : 0u, x, xb, yb, pk, D,
+                                               low_pool, high_pool, mask_pool, longbm);
         // (c): the run's items in the extension window [w0 + T, w0 + 2T) are
         // a prefix of it (the items are sorted by cell).  When fewer than W
         // of the W items loaded after the window belong to the run, they are
         // all of it; otherwise the extension window is read.
+        uint32_t fr_e = 0u, hpos_e = 0u;
         if (ext) {
             const uint64_t he = shfl64w<W>(hp, W - 1);
             uint32_t e[7];
@@ -1169,18 +1160,71 @@ __global__ __launch_bounds__(SEG_THREADS) void k_segw(
             add_words(e, sum);
             const uint64_t len = w0 + T + cnt - (he - 1u);
             const bool all_in = cnt == T;
-            uint32_t fr = 0u;
             if (gl == 0) {
                 if (all_in) {
                     set_long<MULTI>(longbm, D, sid, b);
-                    fr = FR_LONG | (ye.w <= QS_LINK_START ? FR_UNLINKED : 0u);
+                    fr_e = FR_LONG | (ye.w <= QS_LINK_START ? FR_UNLINKED : 0u);
                 } else {
-                    fr = apply_run<MULTI>(true, ce, len, e, xe, ye, pk, D, low_pool, high_pool, mask_pool, longbm);
+                    fr_e = apply_run<MULTI>(true, ce, len, e, xe, ye, pk, D, low_pool, high_pool, mask_pool, longbm);
                 }
             }
-            const uint32_t hpos = (uint32_t)(he - 1u);
-            wave_append((fr & (FR_RISKY | FR_UNLINKED)) != 0u, hpos, ord, &st->n_ord);
-            wave_append((fr & FR_LONG) != 0u, hpos, lng, &st->n_long);
+            hpos_e = (uint32_t)(he - 1u);
+        }
+        // The window's list entries (segment heads): the replay list takes the
+        // risky runs and the long unlinked ones, the long list the long ones.
+        // One 64-bit atomic on the adjacent counters (n_ord, n_long) reserves
+        // the whole wavefront's entries; neither count can carry into the
+        // other (each list holds at most one chunk, under 2^31 entries).
+        const uint32_t hpos_a = (uint32_t)(head - 1u), hpos_b = (uint32_t)(hp - 1u);
+        const bool oa = (fr_a & (FR_RISKY | FR_UNLINKED)) != 0u, la = (fr_a & FR_LONG) != 0u;
+        const bool ob = (fr_b & (FR_RISKY | FR_UNLINKED)) != 0u, lb = (fr_b & FR_LONG) != 0u;
+        const bool oe = (fr_e & (FR_RISKY | FR_UNLINKED)) != 0u, le = (fr_e & FR_LONG) != 0u;
+        uint32_t n_o = (uint32_t)oa + (uint32_t)ob + (uint32_t)oe;
+#pragma unroll
+        for (int j = 0; j < NI; ++j)
+            n_o += (fr_i[j] & FR_RISKY) != 0u ? 1u : 0u;
+        const uint32_t n_l = (uint32_t)la + (uint32_t)lb + (uint32_t)le;
+        static_assert(IPL + 1 < 16, "replay entries per lane fit 4 bits");
+        // Prefix counts from one ballot per count bit: lanes of groups that
+        // skipped this window are inactive here and count as zero.
+        if (__ballot((n_o | n_l) != 0u)) {
+            const uint64_t below = (1ull << lane) - 1ull;
+            uint32_t eo = 0, el = 0, to = 0, tl = 0;
+#pragma unroll
+            for (int bit = 0; bit < 4; ++bit) {
+                const uint64_t m = __ballot(((n_o >> bit) & 1u) != 0u);
+                eo += (uint32_t)__popcll(m & below) << bit;
+                to += (uint32_t)__popcll(m) << bit;
+            }
+#pragma unroll
+            for (int bit = 0; bit < 2; ++bit) {
+                const uint64_t m = __ballot(((n_l >> bit) & 1u) != 0u);
+                el += (uint32_t)__popcll(m & below) << bit;
+                tl += (uint32_t)__popcll(m) << bit;
+            }
+            const uint32_t leader = (uint32_t)__builtin_ctzll(__ballot(true));
+            unsigned long long base = 0;
+            if (lane == leader)
+                base = atomicAdd(reinterpret_cast<unsigned long long *>(&st->n_ord),
+                                 ((unsigned long long)tl << 32) | to);
+            base = shfl64(base, (int)leader);
+            uint32_t po = (uint32_t)base + eo, pl = (uint32_t)(base >> 32) + el;
+            if (oa)
+                ord[po++] = hpos_a;
+#pragma unroll
+            for (int j = 0; j < NI; ++j)
+                if ((fr_i[j] & FR_RISKY) != 0u)
+                    ord[po++] = ipos[j];
+            if (ob)
+                ord[po++] = hpos_b;
+            if (oe)
+                ord[po] = hpos_e;
+            if (la)
+                lng[pl++] = hpos_a;
+            if (lb)
+                lng[pl++] = hpos_b;
+            if (le)
+                lng[pl] = hpos_e;
         }
     }
 }
@@ -1463,7 +1507,9 @@ __global__ __launch_bounds__(256) void k_assign(const qs_trig *__restrict__ trig
     for (uint32_t r = gw; r < n_trig; r += nw) {
         const qs_trig *tr = trig + order[r];
         const uint32_t sid = tr->sid;
-        const uint32_t start = group_start(skey, n_trig, sid << ib);
+        // sketch 0's keys sort first: its group starts at 0 (every trigger of a
+        // single sketch skips the search)
+        const uint32_t start = sid == 0 ? 0u : group_start(skey, n_trig, sid << ib);
         const qs_sdesc &d = D[sid];
         const uint32_t id = next[sid] + (r - start);
         const uint32_t row = id - QS_LINK_START;

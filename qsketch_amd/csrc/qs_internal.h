@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstddef>
 #include <random>
 
 #include "../../include/qsketch.h"
@@ -248,6 +249,9 @@ struct qs_dev_state {
     unsigned long long long_segments; // segments handled by the tail kernel
     unsigned long long chunks_ord;    // chunks with at least one replayed segment
 };
+// k_segw reserves entries of both lists with one 64-bit atomic on (n_ord, n_long)
+static_assert(offsetof(qs_dev_state, n_ord) % 8 == 0 && offsetof(qs_dev_state, n_long) == offsetof(qs_dev_state, n_ord) + 4,
+              "n_ord and n_long form one aligned 64-bit word");
 
 // One allocation found by the in-order replay (filled before ids are known).
 struct qs_trig {
