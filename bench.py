@@ -27,7 +27,7 @@ QUERY_BYTES_PER_KEY = 40    # u32 key + 32 B bucket read + 4 B estimate
 # HBM bytes per key the insert path moved on the default workload, from the
 # FETCH_SIZE / WRITE_SIZE counter runs calibrated with a 1 GiB copy
 # (profiles/insert_query_plain_zipf1.1_2e28_traffic.md)
-INSERT_TRAFFIC_BYTES_PER_KEY = 52.5
+INSERT_TRAFFIC_BYTES_PER_KEY = 49.5
 DEFAULT_WORKLOAD = dict(log2n=28, log2u=26, alpha=1.1, wlf=0.5, n_high=1 << 18)
 
 
