@@ -88,6 +88,7 @@ struct qs_hot {
     hipStream_t side;          // the candidate phase of a plain insert runs here (default priority)
     hipEvent_t ev_fork;        // the caller's stream up to the call
     hipEvent_t ev_cand;        // the candidate phase done
+    bool prev_flat;            // the previous call's sample showed no skew
 };
 
 // The host reads a few device words without a copy or a synchronisation
@@ -701,44 +702,73 @@ int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stre
     qs_evpair e1, e2;
     const bool timed = s->timing != 0;
 
-    // The insert scratch is sized first for every key after the prefix, the
-    // most the cold part can hold: the cold count varies a little between
-    // calls (the hot table's placement races), and growing the scratch
-    // between launches would free and allocate.
-    {
-        const uint64_t offs_r[2] = {0, rest};
-        const uint64_t cap = qs_chunk_keys(s, offs_r);
-        rc = qs_ensure_scratch(s, rest < cap ? rest : cap);
+    // The insert scratch is sized before the prefix is queued, for the whole
+    // call (at least every key after the prefix, the most the cold part can
+    // hold): the cold count varies a little between calls (the hot table's
+    // placement races), and a call without skew after one with skew runs as
+    // one chunk; growing the scratch between launches, or in the next call,
+    // would free and allocate.
+    auto size_scratch = [&]() -> int {
+        const uint64_t offs_n[2] = {0, n};
+        const uint64_t cap = qs_chunk_keys(s, offs_n);
+        return qs_ensure_scratch(s, n < cap ? n : cap);
+    };
+    const uint64_t offs_p[2] = {0, HOT_PREFIX};
+    bool skewed = false;
+    if (!h->prev_flat) {
+        // 2. the prefix goes through the ordinary pipeline on the caller's
+        // stream while 1. the candidates run on the side stream, from the
+        // point the caller's stream had reached at this call (the two share
+        // no memory but the keys, which both only read).  The prefix is
+        // queued first: its launches are what keeps the GPU busy while the
+        // host queues the candidates' short kernels.  The prefix is inserted
+        // whatever the sample shows: a prefix and then the rest is the
+        // sequential result, as for any call split into chunks.
+        rc = size_scratch();
         if (rc)
             return rc;
-    }
-    // 2. the prefix goes through the ordinary pipeline on the caller's
-    // stream while 1. the candidates run on the side stream, from the point
-    // the caller's stream had reached at this call (the two share no memory
-    // but the keys, which both only read).  The prefix is queued first: its
-    // launches are what keeps the GPU busy while the host queues the
-    // candidates' short kernels.  The prefix is inserted whatever the sample
-    // shows: a prefix and then the rest is the sequential result, as for any
-    // call split into chunks.
-    CK(hipEventRecord(h->ev_fork, stream));
-    *used = 1;
-    const uint64_t offs_p[2] = {0, HOT_PREFIX};
-    rc = qs_insert_set(s, keys, offs_p, nullptr, stream, false, false);
-    if (rc)
-        return rc;
-    CK(hipStreamWaitEvent(h->side, h->ev_fork, 0));
-    rc = hot_cand_launch(h, keys, h->side);
-    if (rc)
-        return rc;
-    CK(hipEventRecord(h->ev_cand, h->side));
-    bool skewed = false;
-    rc = hot_cand_result(h, h->side, &skewed); // the GPU runs the prefix meanwhile
-    if (rc)
-        return rc;
-    if (!skewed) { // not skewed enough to pay for the filter: the rest the ordinary way
-        CK(hipStreamWaitEvent(stream, h->ev_cand, 0));
-        const uint64_t offs_r[2] = {0, rest};
-        return qs_insert_set(s, keys + HOT_PREFIX, offs_r, nullptr, stream, true, false);
+        CK(hipEventRecord(h->ev_fork, stream));
+        *used = 1;
+        rc = qs_insert_set(s, keys, offs_p, nullptr, stream, false, false);
+        if (rc)
+            return rc;
+        CK(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+        rc = hot_cand_launch(h, keys, h->side);
+        if (rc)
+            return rc;
+        CK(hipEventRecord(h->ev_cand, h->side));
+        rc = hot_cand_result(h, h->side, &skewed); // the GPU runs the prefix meanwhile
+        if (rc)
+            return rc;
+        h->prev_flat = !skewed;
+        if (!skewed) { // not skewed enough to pay for the filter: the rest the ordinary way
+            CK(hipStreamWaitEvent(stream, h->ev_cand, 0));
+            const uint64_t offs_r[2] = {0, rest};
+            return qs_insert_set(s, keys + HOT_PREFIX, offs_r, nullptr, stream, true, false);
+        }
+    } else {
+        // The previous call showed no skew: 1. the candidates first, on the
+        // caller's stream, so that a call without skew again runs the
+        // ordinary pipeline as one chunk (split at the prefix, every bucket
+        // the prefix touches would be read and written once more: 0.7 ms
+        // of a 2^28-key Zipf 0.8 call).  Either order gives the same state.
+        rc = hot_cand_launch(h, keys, stream);
+        if (rc)
+            return rc;
+        rc = hot_cand_result(h, stream, &skewed);
+        if (rc)
+            return rc;
+        h->prev_flat = !skewed;
+        if (!skewed)
+            return QS_OK; // *used = 0: the caller runs the ordinary insert
+        rc = size_scratch();
+        if (rc)
+            return rc;
+        *used = 1;
+        rc = qs_insert_set(s, keys, offs_p, nullptr, stream, false, false); // 2. the prefix
+        if (rc)
+            return rc;
+        CK(hipEventRecord(h->ev_cand, stream)); // (the wait below is then already met)
     }
     const uint32_t *cand = h->cand;
 

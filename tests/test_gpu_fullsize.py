@@ -93,3 +93,40 @@ def test_hot_filter_same_state_as_ordered_insert(gpu, alpha, log2u):
     uq = np.unique(np.concatenate([k1, k2]))
     est = a.search(torch.from_numpy(uq.view(np.int32)).cuda()).cpu().numpy().view(np.uint32)
     assert np.array_equal(est, o.query(uq, nthreads=_threads()))
+
+
+def test_hot_filter_when_the_skew_changes_between_calls(gpu):
+    """One sketch, 2^26-key calls: uniform, Zipf 1.1, uniform, uniform, Zipf
+    1.1.  After a call without skew the next call finds its candidates before
+    it inserts anything (and inserts a call without skew as one chunk); after
+    any other call it finds them beside the prefix (and inserts the rest of a
+    call without skew after the prefix).  The sequence takes each of the four
+    cases; after every call the whole state equals the ordinary insert's."""
+    import torch
+    import qsketch_amd as q
+    print("CASE hot_filter_skew_changes", flush=True)
+    N = 1 << 26
+    z = q.Zipf(1.1, 1 << 22, device=False).keys_u32_host(5, N)
+    u = q.uniform_u32_host(6, N)
+    n_low = q.number_of_buckets(N, 8, 0.5)
+    a = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, 1 << 18, 32, rng_seed=4)
+    b = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, 1 << 18, 32, rng_seed=4)
+    b.set_hot_filter(False)
+    dz = torch.from_numpy(z.view(np.int32)).cuda()
+    du = torch.from_numpy(u.view(np.int32)).cuda()
+    hot_before = 0
+    for i, (name, d) in enumerate([("uniform", du), ("zipf", dz), ("uniform", du), ("uniform", du), ("zipf", dz)]):
+        print(f"  call {i}: {name}", flush=True)
+        a.insert(d)
+        b.insert(d)
+        sa, sb = a.export_state(), b.export_state()
+        for f in ("low", "high", "next"):
+            assert np.array_equal(sa[f], sb[f]), (i, name, f)
+        hot = a.stats()["hot_keys"]
+        if name == "zipf":
+            assert hot - hot_before > N // 2, (i, hot - hot_before)  # the skewed calls took the filter
+        else:
+            assert hot == hot_before, i
+        hot_before = hot
+    assert a.stats()["keys_inserted"] == 5 * N
+    assert b.stats()["hot_keys"] == 0
