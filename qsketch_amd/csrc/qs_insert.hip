@@ -2058,7 +2058,9 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
     }
     hipLaunchKernelGGL(k_trig_keys, dim3(grid_cap(((uint64_t)sc.trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
                        sc.trig, s->d_st, sc.trig_cap, s->pk.ib, sc.tk[0], sc.tv[0]);
-    const int tbits = (int)(s->pk.sb + s->pk.ib);
+    // a single sketch's trigger keys are stream indices below cn (one digit
+    // pass fewer for a chunk of at most 2^24 keys)
+    const int tbits = s->nsk == 1 ? (int)qs_bits_for(cn) : (int)(s->pk.sb + s->pk.ib);
     uint32_t *tk[2] = {sc.tk[0], sc.tk[1]}, *tv[2] = {sc.tv[0], sc.tv[1]};
     const int tc = qs_sort_pairs_dev(tk, tv, sc.hist, &s->d_st->n_trig, sc.trig_cap,
                                      tbits > 32 ? 32 : tbits, stream);
