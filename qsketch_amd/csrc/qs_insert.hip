@@ -1323,14 +1323,19 @@ __global__ __launch_bounds__(256) void k_ordered(const uint64_t *__restrict__ IT
     const uint32_t n_ord = st->n_ord;
     uint32_t ntb = 0, nlb = 0;
     unsigned long long replayed_all = 0;
-    // The next segment's head and the next batch's items are loaded ahead,
-    // unconditionally from clamped positions, so that waits do not cover them.
+    // Loaded ahead, unconditionally from clamped positions so that waits do
+    // not cover them: the next segment's first batch of items, the head of
+    // the segment after it, and the next batch of this segment's items.
     const uint32_t last = n - 1u, olast = n_ord ? n_ord - 1u : 0u;
-    uint32_t p0n = ord[gw < olast ? gw : olast];
+    auto head_at = [&](uint32_t h) -> uint32_t { return ord[h < olast ? h : olast]; };
+    uint32_t p0n = head_at(gw), p0nn = head_at(gw + nw);
+    uint64_t iqn = IT[p0n + lane < last ? p0n + lane : last];
     for (uint32_t h = gw; h < n_ord; h += nw) {
         const uint32_t p0 = p0n;
-        p0n = ord[h + nw < olast ? h + nw : olast];
-        uint64_t iq = IT[p0 + lane < last ? p0 + lane : last]; // the first batch; lane 0 holds the head
+        uint64_t iq = iqn; // the first batch; lane 0 holds the head
+        p0n = p0nn;
+        p0nn = head_at(h + 2u * nw);
+        iqn = IT[p0n + lane < last ? p0n + lane : last];
         const uint64_t cell = it_cell(shfl64(iq, 0), pk);
         const uint32_t b = (uint32_t)(cell & pk.bmask);
         const uint32_t sid = (uint32_t)(cell >> pk.bb);
