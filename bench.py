@@ -483,7 +483,10 @@ def ceilings(reps=5, copy_gib=(1, 4), gathers=(16, 32, 64, 128)):
       * a streaming fill (write only) of 4 GiB, with plain and with
         non-temporal stores (the clear's pattern);
       * 2^25 random reads of 16, 32, 64 and 128 bytes from a 6 GiB buffer,
-        each writing a 4-byte result (the query's pattern at 32 bytes)."""
+        each writing a 4-byte result (the query's pattern at 32 bytes);
+      * 2^24 read-modify-writes of 32 bytes in the same buffer, at random
+        lines and in ascending order with random gaps (the segment pass's
+        pattern: 16.5 M distinct buckets of 201 M, in bucket order)."""
     import numpy as np
     import torch
     import qsketch_amd as q
@@ -503,7 +506,7 @@ def ceilings(reps=5, copy_gib=(1, 4), gathers=(16, 32, 64, 128)):
             ts.append(e0.elapsed_time(e1) * 1e-3)
         return float(np.median(ts))
 
-    out = {"copy": {}, "fill": {}, "gather": {}}
+    out = {"copy": {}, "fill": {}, "gather": {}, "rmw": {}}
     for gib in copy_gib:
         src = torch.ones(gib << 28, dtype=torch.int32, device=dev)
         dst = torch.empty_like(src)
@@ -524,6 +527,12 @@ def ceilings(reps=5, copy_gib=(1, 4), gathers=(16, 32, 64, 128)):
             out["gather"][f"{g}B"] = {"seconds": round(t, 6), "greads_s": round(n / t / 1e9, 2),
                                       "gb_s_read": round(n * g / t / 1e9, 1),
                                       "gb_s_algorithmic": round(n * (g + 4) / t / 1e9, 1)}
+        nr = 1 << 24
+        for sorted_ in (0, 1):
+            t = timed(lambda: q.check(L.qs_calib_rmw(buf.data_ptr(), 6 << 30, 32, nr, 11, sorted_, sp)))
+            out["rmw"]["32B_sorted" if sorted_ else "32B_random"] = {
+                "seconds": round(t, 6), "g_rmw_s": round(nr / t / 1e9, 2),
+                "gb_s_algorithmic": round(nr * 64 / t / 1e9, 1)}
         del buf, res
     torch.cuda.empty_cache()
     return out

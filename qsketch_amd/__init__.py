@@ -171,6 +171,8 @@ _SIGS = {
     "qs_calib_fill": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_void_p]),
     "qs_calib_gather": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p,
                                   C.c_void_p]),
+    "qs_calib_rmw": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int,
+                                C.c_void_p]),
     # generators (include/qsketch_gen.h)
     "qs_zipf_create": (C.c_int, [C.c_double, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
     "qs_zipf_destroy": (None, [C.c_void_p]),

@@ -7,6 +7,9 @@
 //   k_calib_gather  n random reads of G bytes (G / 16 lanes, 16 bytes each)
 //                   from a buffer of G-byte lines; the group's xor goes to a
 //                   4-byte result per read, like a query's estimate
+//   k_calib_rmw     n read-modify-writes of G bytes (every word + 1), at
+//                   random lines or in ascending order with random gaps (the
+//                   segment pass's pattern: distinct buckets, sorted)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -62,6 +65,25 @@ __global__ __launch_bounds__(256) void k_calib_gather(const cu32x4 *__restrict__
     }
 }
 
+template <int LANES>
+__global__ __launch_bounds__(256) void k_calib_rmw(cu32x4 *__restrict__ buf, uint64_t n_lines, uint64_t n,
+                                                   uint64_t seed, int sorted)
+{
+    const uint32_t part = threadIdx.x % LANES;
+    const uint64_t per_block = 256 / LANES;
+    const uint64_t stride = (uint64_t)gridDim.x * per_block;
+    const uint64_t gap = n_lines / n; // >= 1: the host checks n <= n_lines
+    for (uint64_t i = (uint64_t)blockIdx.x * per_block + threadIdx.x / LANES; i < n; i += stride) {
+        const uint64_t h = calib_mix(seed, i);
+        // sorted: line i * gap + (h mod gap) <= n * gap - 1 < n_lines
+        const uint64_t line = sorted ? i * gap + h % gap : __umul64hi(h, n_lines);
+        cu32x4 *p = buf + line * LANES + part;
+        cu32x4 v = *p;
+        v += 1u;
+        *p = v;
+    }
+}
+
 extern "C" int qs_calib_copy(const void *d_src, void *d_dst, uint64_t bytes, void *stream)
 {
     if (!d_src || !d_dst || bytes % 16)
@@ -111,6 +133,32 @@ extern "C" int qs_calib_gather(const void *d_buf, uint64_t buf_bytes, uint32_t a
     case 2: hipLaunchKernelGGL(k_calib_gather<2>, grid, block, 0, st, b, n_lines, n, seed, d_out); break;
     case 4: hipLaunchKernelGGL(k_calib_gather<4>, grid, block, 0, st, b, n_lines, n, seed, d_out); break;
     default: hipLaunchKernelGGL(k_calib_gather<8>, grid, block, 0, st, b, n_lines, n, seed, d_out); break;
+    }
+    return hipGetLastError() == hipSuccess ? QS_OK : QS_ERR_HIP;
+}
+
+extern "C" int qs_calib_rmw(void *d_buf, uint64_t buf_bytes, uint32_t access_bytes, uint64_t n, uint64_t seed,
+                            int sorted, void *stream)
+{
+    if (!d_buf || (access_bytes != 16 && access_bytes != 32 && access_bytes != 64 && access_bytes != 128))
+        return QS_ERR_PARAM;
+    const uint64_t n_lines = buf_bytes / access_bytes;
+    if (n > n_lines)
+        return QS_ERR_PARAM;
+    if (n == 0)
+        return QS_OK;
+    const uint32_t lanes = access_bytes / 16;
+    uint64_t blocks = (n * lanes + 255) / 256;
+    if (blocks > 256 * 8)
+        blocks = 256 * 8;
+    const dim3 grid((uint32_t)blocks), block(256);
+    const hipStream_t st = (hipStream_t)stream;
+    cu32x4 *b = (cu32x4 *)d_buf;
+    switch (lanes) {
+    case 1: hipLaunchKernelGGL(k_calib_rmw<1>, grid, block, 0, st, b, n_lines, n, seed, sorted); break;
+    case 2: hipLaunchKernelGGL(k_calib_rmw<2>, grid, block, 0, st, b, n_lines, n, seed, sorted); break;
+    case 4: hipLaunchKernelGGL(k_calib_rmw<4>, grid, block, 0, st, b, n_lines, n, seed, sorted); break;
+    default: hipLaunchKernelGGL(k_calib_rmw<8>, grid, block, 0, st, b, n_lines, n, seed, sorted); break;
     }
     return hipGetLastError() == hipSuccess ? QS_OK : QS_ERR_HIP;
 }
