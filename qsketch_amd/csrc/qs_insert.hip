@@ -1614,7 +1614,10 @@ __global__ __launch_bounds__(256) void k_tail(const uint64_t *__restrict__ IT, u
 {
     if (st->n_long == 0)
         return;
-    __shared__ uint32_t s_masks[MULTI ? 1 : QS_MAX_MASKS];
+    // a single sketch's masks, n_masks0 words of dynamic LDS (sized by the
+    // launch: 16 KiB for the largest table would hold the kernel to five
+    // wavefronts per SIMD)
+    extern __shared__ uint32_t s_masks[];
     if (!MULTI)
         for (uint32_t i = threadIdx.x; i < n_masks0; i += blockDim.x)
             s_masks[i] = mask_pool[i];
@@ -2080,14 +2083,15 @@ static int insert_chunk(qs_sketch *s, const KeyT<KB> *keys, uint32_t cn, uint64_
     // k_segw windows under 64 items mark runs long that fit inside one step
     const bool short_runs = s->seg_w != 0 && s->seg_w * s->seg_ipl < 64u;
     const dim3 tgrid(grid_cap(((uint64_t)cn + TAIL_WINDOW * 4 - 1) / (TAIL_WINDOW * 4), 8));
+    const size_t tail_lds = MULTI ? 0 : (size_t)nm0 * sizeof(uint32_t); // nm0 <= QS_MAX_MASKS
     if (s->pk.bb + s->pk.sb <= 31u)
-        hipLaunchKernelGGL((k_tail<MULTI, uint32_t>), tgrid, dim3(256), 0, stream, IT, cn, s->d_desc, nm0, s->d_low,
-                           s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, sc.lng, short_runs,
-                           s->d_st);
+        hipLaunchKernelGGL((k_tail<MULTI, uint32_t>), tgrid, dim3(256), tail_lds, stream, IT, cn, s->d_desc, nm0,
+                           s->d_low, s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, sc.lng,
+                           short_runs, s->d_st);
     else
-        hipLaunchKernelGGL((k_tail<MULTI, uint64_t>), tgrid, dim3(256), 0, stream, IT, cn, s->d_desc, nm0, s->d_low,
-                           s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, sc.lng, short_runs,
-                           s->d_st);
+        hipLaunchKernelGGL((k_tail<MULTI, uint64_t>), tgrid, dim3(256), tail_lds, stream, IT, cn, s->d_desc, nm0,
+                           s->d_low, s->d_high, s->d_masks, s->pk, s->d_longbm, s->d_tail_of, s->d_next, sc.lng,
+                           short_runs, s->d_st);
     hipLaunchKernelGGL(k_clear_long, dim3(grid_cap(((uint64_t)cn / 64 + 255) / 256, 4)), dim3(256), 0, stream,
                        sc.lng, IT, s->d_desc, s->pk, s->d_longbm, s->d_st);
     hipLaunchKernelGGL(k_next, dim3(grid_cap(((uint64_t)sc.trig_cap + 255) / 256, 4)), dim3(256), 0, stream,
