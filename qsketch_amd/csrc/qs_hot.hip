@@ -506,22 +506,27 @@ __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ a, u
 
 // the hot counts summed over the persistent blocks and added to the high
 // rows: a block takes 64 slots, its four wavefronts a quarter of the rows each
-__global__ __launch_bounds__(256) void k_hot_flush(const uint32_t *__restrict__ part, uint32_t nblk,
-                                                   const uint32_t *__restrict__ tkey,
-                                                   const uint32_t *__restrict__ tinfo,
-                                                   const uint32_t *__restrict__ tmask, uint32_t *__restrict__ high)
+#define FLUSH_WAVES 16u // wavefronts per block, each summing every 16th row of part
+__global__ __launch_bounds__(64 * FLUSH_WAVES) void k_hot_flush(const uint32_t *__restrict__ part, uint32_t nblk,
+                                                                const uint32_t *__restrict__ tkey,
+                                                                const uint32_t *__restrict__ tinfo,
+                                                                const uint32_t *__restrict__ tmask,
+                                                                uint32_t *__restrict__ high)
 {
-    __shared__ uint32_t s_sum[4][64];
+    __shared__ uint32_t s_sum[FLUSH_WAVES][64];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t s = blockIdx.x * 64u + lane;
     uint32_t c = 0;
-    for (uint32_t b = wv; b < nblk; b += 4u)
+    for (uint32_t b = wv; b < nblk; b += FLUSH_WAVES)
         c += part[(size_t)b * HOT_SLOTS + s];
     s_sum[wv][lane] = c;
     __syncthreads();
     if (wv != 0)
         return;
-    c = s_sum[0][lane] + s_sum[1][lane] + s_sum[2][lane] + s_sum[3][lane];
+    c = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < FLUSH_WAVES; ++w)
+        c += s_sum[w][lane];
     if (!c || tkey[s] == HOT_EMPTY)
         return;
     uint32_t *g = high + (size_t)(tinfo[s] - QS_LINK_START) * 32u;
@@ -787,7 +792,7 @@ int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stre
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, h->bsum, nb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, stream, h->ccold, nchunks, h->bsum, nb);
     hipLaunchKernelGGL(k_hot_move, dim3(h->nblk * 8u), dim3(256), 0, stream, h->stage, nchunks, h->ccold, h->cold);
-    hipLaunchKernelGGL(k_hot_flush, dim3(HOT_SLOTS / 64), dim3(256), 0, stream, h->part, h->nblk, h->tkey,
+    hipLaunchKernelGGL(k_hot_flush, dim3(HOT_SLOTS / 64), dim3(64 * FLUSH_WAVES), 0, stream, h->part, h->nblk, h->tkey,
                        h->tinfo, h->tmask, s->d_high);
     CK(hipGetLastError());
     if (timed)

@@ -669,22 +669,29 @@ __global__ __launch_bounds__(SF_THREADS) void k_sf_rows(const KeyT<KB> *__restri
 // summed over the persistent blocks and added with sf_add (after the cold
 // keys' kernels, so no plain write of k_sf_rows runs beside these atomics).
 // A block takes 64 slots, its four wavefronts a quarter of the rows each.
-__global__ __launch_bounds__(256) void k_sf_hot_flush(const uint32_t *__restrict__ part, uint32_t nblk,
-                                                      const uint32_t *__restrict__ tkey, uint32_t nslots, sf_par p,
-                                                      const uint32_t *__restrict__ index,
-                                                      uint32_t *__restrict__ split, uint32_t *__restrict__ search)
+#define SF_FLUSH_WAVES 16u // wavefronts per block, each summing every 16th row of part
+__global__ __launch_bounds__(64 * SF_FLUSH_WAVES) void k_sf_hot_flush(const uint32_t *__restrict__ part,
+                                                                      uint32_t nblk,
+                                                                      const uint32_t *__restrict__ tkey,
+                                                                      uint32_t nslots, sf_par p,
+                                                                      const uint32_t *__restrict__ index,
+                                                                      uint32_t *__restrict__ split,
+                                                                      uint32_t *__restrict__ search)
 {
-    __shared__ uint32_t s_sum[4][64];
+    __shared__ uint32_t s_sum[SF_FLUSH_WAVES][64];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t sl = blockIdx.x * 64u + lane;
     uint32_t c = 0;
-    for (uint32_t b = wv; b < nblk; b += 4u)
+    for (uint32_t b = wv; b < nblk; b += SF_FLUSH_WAVES)
         c += part[(size_t)b * nslots + sl];
     s_sum[wv][lane] = c;
     __syncthreads();
     if (wv != 0)
         return;
-    c = s_sum[0][lane] + s_sum[1][lane] + s_sum[2][lane] + s_sum[3][lane];
+    c = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < SF_FLUSH_WAVES; ++w)
+        c += s_sum[w][lane];
     if (c) // a slot without a key counts nothing
         sf_add<4>(split, search, p, index, tkey[sl], c);
 }
@@ -901,8 +908,8 @@ static int sf_insert_hot(qs_sf *s, const sf_par &p, const uint32_t *keys, uint32
         if (rc)
             return rc;
     }
-    hipLaunchKernelGGL(k_sf_hot_flush, dim3(hs.nslots / 64u), dim3(256), 0, 0, hs.part, hs.nblk, hs.tkey, hs.nslots,
-                       p, s->d_index, s->d_split, s->d_search);
+    hipLaunchKernelGGL(k_sf_hot_flush, dim3(hs.nslots / 64u), dim3(64 * SF_FLUSH_WAVES), 0, 0, hs.part, hs.nblk,
+                       hs.tkey, hs.nslots, p, s->d_index, s->d_split, s->d_search);
     CK(hipGetLastError());
     return QS_OK;
 }
