@@ -757,7 +757,14 @@ int qs_insert_hot(qs_sketch *s, const void *d_keys, uint64_t n, hipStream_t stre
         // ordinary pipeline as one chunk (split at the prefix, every bucket
         // the prefix touches would be read and written once more: 0.7 ms
         // of a 2^28-key Zipf 0.8 call).  Either order gives the same state.
+        // The candidates are timed with the filter here: the call waits for
+        // them.
+        qs_evpair e0;
+        if (timed)
+            qs_tim_begin(s, e0, QS_TK_HOT, 0, stream);
         rc = hot_cand_launch(h, keys, stream);
+        if (timed)
+            qs_tim_end(s, e0, stream);
         if (rc)
             return rc;
         rc = hot_cand_result(h, stream, &skewed);
