@@ -200,8 +200,8 @@ def main():
     counts = torch.empty_like(uniq)
     n_low = q.number_of_buckets(N, 8, a.wlf)
     sk = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, a.n_high, 32, 1, 1021, rng_seed=1)
+    sk.set_hot_filter(not a.no_hot_filter)  # the tuner times the insert as it will run
     tune = None if a.no_tune else sk.autotune(keys, reps=1)
-    sk.set_hot_filter(not a.no_hot_filter)
     torch.cuda.synchronize()
 
     def step():

@@ -257,7 +257,9 @@ void qs_graph_destroy(qs_graph *g);
  * one insert batch with all 16 shapes on a twin set (the caller's set is not
  * modified; the twin needs as much device memory as the set), checks that all
  * of them leave the same state, and keeps the shape with the fastest segment
- * pass on the set (QS_ERR_STATE, shape unchanged, if two shapes disagree).   */
+ * pass on the set (QS_ERR_STATE, shape unchanged, if two shapes disagree).
+ * A single sketch's batch is inserted as qs_insert would (through the hot-key
+ * filter when the sketch has it on and the batch takes it).                 */
 typedef struct qs_tune_entry {
     uint32_t width, items_per_lane;
     double seg_ms;     /* segment pass, mean over reps */

@@ -130,3 +130,30 @@ def test_hot_filter_when_the_skew_changes_between_calls(gpu):
         hot_before = hot
     assert a.stats()["keys_inserted"] == 5 * N
     assert b.stats()["hot_keys"] == 0
+
+
+def test_autotune_of_a_filtered_insert_keeps_one_state(gpu):
+    """The autotuner of a single sketch inserts a 2^26-key Zipf batch through
+    the hot-key filter (as the sketch's own insert would) with each of the 16
+    segment-pass shapes on a twin sketch: every shape must leave the same
+    state (a checksum per shape; QS_ERR_STATE otherwise), and the sketch's
+    own insert afterwards equals the unfiltered one."""
+    import torch
+    import qsketch_amd as q
+    print("CASE autotune_filtered", flush=True)
+    N = 1 << 26
+    z = q.Zipf(1.1, 1 << 22, device=False).keys_u32_host(7, N)
+    d = torch.from_numpy(z.view(np.int32)).cuda()
+    n_low = q.number_of_buckets(N, 8, 0.5)
+    a = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, 1 << 18, 32, rng_seed=5)
+    b = q.Sketch_GPU_Sub_Warp_Level(n_low, 8, 1 << 18, 32, rng_seed=5)
+    b.set_hot_filter(False)
+    r = a.autotune(d, reps=1)
+    assert r["consistent"] and len(r["table"]) == 16
+    assert a.stats()["keys_inserted"] == 0  # the twin took the inserts
+    a.insert(d)
+    b.insert(d)
+    assert a.stats()["hot_keys"] > N // 2
+    sa, sb = a.export_state(), b.export_state()
+    for f in ("low", "high", "next"):
+        assert np.array_equal(sa[f], sb[f]), f
