@@ -86,8 +86,8 @@ def cpu_info() -> dict:
 
 def host_threads():
     """Threads for the all-cores variant: every core of the affinity mask, but
-    no more than OMP_NUM_THREADS when that is set (the GPU box gives one GPU's
-    job a share of the host's cores and says so there)."""
+    no more than OMP_NUM_THREADS when that is set (a host shared between
+    GPUs may give one GPU's job a share of its cores and say so there)."""
     aff = len(os.sched_getaffinity(0))
     env = os.environ.get("OMP_NUM_THREADS", "")
     cap = int(env) if env.isdigit() and int(env) > 0 else aff
@@ -163,8 +163,8 @@ def cpu_baseline(keys_dev, n_low, n_high, sample_log2):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
-                                   "designed (see DESIGN.md) but not built", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU only; sharding across GPUs is "
+                                   "designed but not built", "n_gpus": a.gpus}))
         sys.exit(2)
     if a.calib_copy:
         import torch

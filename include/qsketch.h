@@ -48,8 +48,8 @@ extern "C" {
 #define QS_LITERAL 0 /* v3:187 as written: trigger insert adds to g[j mod 8] */
 #define QS_PAPER   1 /* doc/main.tex:433 intent: trigger insert adds to g[j]  */
 
-/* Parameter set.  Defaults are lib/default.cu:39-54 and the driver's
- * constructor call src/main.cu:390-391:
+/* Parameter set.  Defaults are lib/default.cu:39-54 and the reference
+ * program's constructor call src/main.cu:390-391:
  *   Sketch_GPU_Sub_Warp_Level(n, 8, 1024, 32, 1, 1021).                     */
 typedef struct qs_params {
     uint64_t n_low;      /* number of 32-byte low buckets (v3:544 _n_low)          */
@@ -73,7 +73,7 @@ typedef struct qs_sketch qs_sketch;
 /* Fill *p with the default parameter set, n_low sized by the reference rule
  * Sketch_GPU::number_of_buckets(expected_keys, 8, wlf)
  * (device_sketch.h:371-376 + find_greatest_prime, lib/qsketch.cu:50-115).
- * Replaces the driver's sizing call, src/main.cu:371-372. */
+ * Replaces the reference program's sizing call, src/main.cu:371-372. */
 int qs_default_params(qs_params *p, uint64_t expected_keys, double wlf);
 
 /* find_greatest_prime (lib/qsketch.cu:50-115). */

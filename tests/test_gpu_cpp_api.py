@@ -1,5 +1,6 @@
 """The C++ interface (include/qsketch.hpp) driven like the reference's Test
-harness, estimates checked bit-exact against the oracle inside the driver."""
+harness, estimates checked bit-exact against the oracle inside the test
+program (tests/native/cpp_api_test.cpp)."""
 import os
 import subprocess
 
