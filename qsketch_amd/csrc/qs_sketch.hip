@@ -556,14 +556,20 @@ extern "C" int qs_clear(qs_sketch *s)
 {
     if (!s)
         return QS_ERR_PARAM;
+    // Blocking, as the reference's clear: every earlier use of the tables has
+    // finished before the fills start, and the fills and the control block's
+    // reset have finished when the call returns.  In between they are queued
+    // back to back on the null stream, with no host round trip between them.
     CK(hipDeviceSynchronize());
-    CK(hipMemset(s->d_low, 0, s->low_words * 4));
-    CK(hipMemset(s->d_high, 0, s->high_words * 4));
-    CK(hipMemset(s->d_longbm, 0, s->bm_words * 4));
-    int rc = reset_state(s);
-    if (rc)
-        return rc;
-    CK(hipDeviceSynchronize());
+    CK(hipMemsetAsync(s->d_low, 0, s->low_words * 4, 0));
+    CK(hipMemsetAsync(s->d_high, 0, s->high_words * 4, 0));
+    CK(hipMemsetAsync(s->d_longbm, 0, s->bm_words * 4, 0));
+    qs_dev_state h;
+    memset(&h, 0, sizeof(h));
+    std::vector<uint32_t> nx(s->nsk, QS_LINK_START + 1);
+    CK(hipMemcpyAsync(s->d_st, &h, sizeof(h), hipMemcpyHostToDevice, 0));
+    CK(hipMemcpyAsync(s->d_next, nx.data(), sizeof(uint32_t) * s->nsk, hipMemcpyHostToDevice, 0));
+    CK(hipDeviceSynchronize()); // (h and nx are read before this returns)
     s->keys_inserted = 0;
     s->chunks = 0;
     s->sticky_err = 0;
