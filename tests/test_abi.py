@@ -275,3 +275,22 @@ def test_sf_create_rejects_bad_parameters():
                  (1021, 8, 3, 1, 0, 4), (1021, 8, 3, 1, 5000, 4), (1021, 8, 3, 1, 1021, 2),
                  (1021, 8, 3, 2, 1021, 8), (0, 8, 3, 1, 1021, 4), (1 << 32, 8, 1, 1, 1021, 8)):
         assert L.qs_sf_create(*args, 1, C.byref(h)) == q.QS_ERR_PARAM, args
+
+
+def test_calibration_kernels_reject_bad_parameters():
+    """The calibration entry points check their arguments on the host before
+    anything is launched (a null buffer, an access size other than 16, 32, 64
+    or 128 bytes, sizes not a multiple of 16, more accesses than lines)."""
+    import qsketch_amd as q
+    L = q.lib()
+    fake = 1 << 20  # never dereferenced: every call below is refused first
+    assert L.qs_calib_copy(None, fake, 1 << 20, None) == q.QS_ERR_PARAM
+    assert L.qs_calib_copy(fake, fake, 17, None) == q.QS_ERR_PARAM
+    assert L.qs_calib_fill(None, 1 << 20, 0, 0, None) == q.QS_ERR_PARAM
+    assert L.qs_calib_fill(fake, 24, 0, 0, None) == q.QS_ERR_PARAM
+    for size in (0, 8, 48, 256):
+        assert L.qs_calib_gather(fake, 1 << 20, size, 1, 1, fake, None) == q.QS_ERR_PARAM, size
+        assert L.qs_calib_rmw(fake, 1 << 20, size, 1, 1, 0, None) == q.QS_ERR_PARAM, size
+    assert L.qs_calib_rmw(None, 1 << 20, 32, 1, 1, 0, None) == q.QS_ERR_PARAM
+    assert L.qs_calib_rmw(fake, 1 << 10, 32, 33, 1, 1, None) == q.QS_ERR_PARAM  # 32 lines, 33 accesses
+    assert L.qs_calib_rmw(fake, 1 << 10, 32, 0, 1, 1, None) == q.QS_OK  # nothing to do
